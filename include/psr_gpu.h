@@ -1,0 +1,186 @@
+/*
+ * psr_gpu.h -- C ABI of the MI355X (gfx950) search hot path of PulsarSr.
+ *
+ * The entry points below replace the reference's per-DM-trial loop
+ *   crates/cli/src/main.rs:462-469  (delays -> apply -> periodicity, rayon par_iter)
+ * and the crate functions it calls.  Each declaration names the reference
+ * interface it stands in for.  Conventions:
+ *   - every function returns PSR_OK (0) or a negative PSR_E_* code, never aborts
+ *     or throws across the ABI; psr_last_error() returns the message of the
+ *     calling thread's last failure (the text mirrors the reference's Error);
+ *   - all host buffers are caller-owned; device memory belongs to the psr_ctx
+ *     and is reused across calls;
+ *   - a psr_ctx is not thread-safe: use one per thread, or serialise calls;
+ *   - there is no CPU fallback: GPU entry points fail with PSR_E_DEVICE when no
+ *     gfx950 device is present.
+ */
+#ifndef PSR_GPU_H
+#define PSR_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSR_OK 0
+#define PSR_E_INVALID (-1)  /* dedisperse::Error: invalid input (crates/dedisperse/src/lib.rs:5-7) */
+#define PSR_E_SHAPE (-2)    /* array shape mismatch / invalid output length (lib.rs:68,76-78) */
+#define PSR_E_DEVICE (-3)   /* HIP runtime error or no usable device */
+#define PSR_E_OOM (-4)      /* device allocation failed or exceeded the ctx memory cap */
+#define PSR_E_FORMAT (-5)   /* fbank::Error::Format (crates/fbank/src/lib.rs:12-18) */
+#define PSR_E_CONTRACT (-6) /* a reference assert! would panic (search/src/lib.rs:34, fold/src/lib.rs:23,89) */
+#define PSR_E_OVERFLOW (-7) /* per-trial peak buffer overflowed; result would not be exact */
+
+#define PSR_PRECISION_FP32 0u /* series and spectra stored FP32 (the reference's arithmetic) */
+#define PSR_PRECISION_FP16 1u /* series and spectra stored FP16 after per-trial scaling */
+
+typedef struct psr_ctx psr_ctx;
+typedef struct psr_obs psr_obs; /* one device-resident observation */
+
+/* Mirrors search::Candidate (crates/search/src/lib.rs:6-13); usize -> uint64_t. */
+typedef struct {
+    double frequency_hz;
+    double period_s;
+    double dm;
+    uint64_t harmonics;
+    double score;
+} psr_candidate;
+
+/* Mirrors search::Config (crates/search/src/lib.rs:14-32) plus two knobs the
+ * reference hard-codes: max_harmonics (16 = the reference's 1,2,4,8,16) and the
+ * storage precision of series/spectra. */
+typedef struct {
+    double min_hz;
+    double max_hz;
+    double threshold;
+    uint64_t max_candidates;
+    const double *birdies; /* nbirdies pairs (center_hz, half_width_hz) */
+    uint64_t nbirdies;
+    uint32_t max_harmonics;
+    uint32_t precision; /* PSR_PRECISION_* */
+} psr_search_cfg;
+
+/* Mirrors fbank::Header's numerical part (crates/fbank/src/lib.rs:29-38). */
+typedef struct {
+    uint32_t nchans;
+    uint32_t nbits;
+    double tsamp;
+    double fch1;
+    double foff;
+    uint64_t data_offset; /* bytes before the payload */
+    uint64_t nsamples;    /* whole frames in the payload */
+} psr_header;
+
+/* A packed SIGPROC payload: time-major frames, LSB-first for 1/2/4-bit
+ * (crates/fbank/src/lib.rs:149-196).  nbits in {1,2,4,8,16,32}. */
+typedef struct {
+    const void *payload;
+    uint64_t payload_bytes;
+    uint32_t nbits;
+    uint32_t nchans;
+    uint64_t nsamples; /* frames to use (<= whole frames in payload) */
+    double tsamp;
+    double fch1;
+    double foff;
+} psr_filterbank;
+
+/* Mirrors fold::Profile's scalar fields (crates/fold/src/lib.rs:3-11). */
+typedef struct {
+    double period_s;
+    double snr;
+    double reduced_chi_square;
+} psr_profile_stats;
+
+/* Per-stage device time of the last psr_search call, milliseconds, measured
+ * with HIP events on the ctx stream the kernels ran on. */
+typedef struct {
+    double total_ms;
+    double delays_ms;
+    double dedisperse_ms;
+    double fft_ms;
+    double whiten_ms;
+    double harmsum_ms;
+    double finalize_ms;
+    uint64_t trials;
+    uint64_t series_len;
+    uint64_t fft_len;
+} psr_stage_times;
+
+/* ---- library / context ------------------------------------------------- */
+const char *psr_version(void);
+/* Message of the calling thread's most recent failure ("" if none). */
+const char *psr_last_error(void);
+/* Create a context on HIP device `device` (flags reserved, pass 0). */
+int psr_ctx_create(int device, uint32_t flags, psr_ctx **out);
+void psr_ctx_destroy(psr_ctx *ctx);
+/* Cap the device memory a ctx may hold (bytes; 0 = unlimited).  Host-forcible OOM path. */
+int psr_ctx_set_memory_cap(psr_ctx *ctx, uint64_t bytes);
+
+/* ---- host math: crates/dedisperse (no GPU needed) ----------------------- */
+/* dedisperse::delays (crates/dedisperse/src/lib.rs:9-34) */
+int psr_delays(const double *freqs, size_t nch, double tsamp, double dm, uint64_t *out);
+/* dedisperse::dm_step (crates/dedisperse/src/lib.rs:35-44) */
+int psr_dm_step(const double *freqs, size_t nch, double tsamp, double *out);
+/* dedisperse::plan (crates/dedisperse/src/lib.rs:45-60); writes *count trials (<= cap). */
+int psr_plan(double min, double max, double step, double *out, size_t cap, size_t *count);
+/* fbank::Header::frequencies (crates/fbank/src/lib.rs:40-44) */
+int psr_frequencies(double fch1, double foff, size_t nch, double *out);
+/* fbank::parse_header (crates/fbank/src/lib.rs:72-147) */
+int psr_parse_header(const uint8_t *bytes, size_t len, psr_header *out);
+/* search::sift (crates/search/src/lib.rs:147-176); returns kept count in *count. */
+int psr_sift(const psr_candidate *in, size_t n, double resolution_hz, size_t limit,
+             int reject_dm_zero, psr_candidate *out, size_t *count);
+
+/* ---- observations (device-resident filterbank) -------------------------- */
+/* fbank::decode's unpacking (crates/fbank/src/lib.rs:149-196) done on the GPU:
+ * uploads the packed payload and lays it out channel-major in HBM. */
+int psr_obs_create(psr_ctx *ctx, const psr_filterbank *fb, psr_obs **out);
+/* An observation from f32 channel-major data: the `data` argument of
+ * dedisperse::apply (crates/dedisperse/src/lib.rs:62-66). */
+int psr_obs_create_f32(psr_ctx *ctx, const float *data, uint64_t nsamples, uint32_t nchans,
+                       double tsamp, double fch1, double foff, psr_obs **out);
+void psr_obs_destroy(psr_obs *obs);
+/* Copy the channel-major f32 samples back (nchans*nsamples floats), for checks. */
+int psr_obs_download_f32(psr_obs *obs, float *out);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* dedisperse::delays + dedisperse::apply for `ntrials` DMs
+ * (crates/dedisperse/src/lib.rs:9-34, 62-90).  length==0 means None (full
+ * available length).  out[ntrials*length] f32; *out_len gets the length used. */
+int psr_dedisperse(psr_ctx *ctx, psr_obs *obs, const double *dms, size_t ntrials, size_t length,
+                   float *out, size_t *out_len);
+
+/* The per-trial loop of crates/cli/src/main.rs:462-469: for every DM, delays ->
+ * apply(Some(length)) -> search::periodicity (crates/search/src/lib.rs:33-145).
+ * Writes up to cap_per_trial candidates per trial at out[t*cap_per_trial], in the
+ * reference's order, and counts[t]. */
+int psr_search(psr_ctx *ctx, psr_obs *obs, const double *dms, size_t ntrials, size_t length,
+               const psr_search_cfg *cfg, psr_candidate *out, size_t cap_per_trial,
+               uint32_t *counts);
+/* Stage timings of the last psr_search on this ctx. */
+int psr_last_stage_times(const psr_ctx *ctx, psr_stage_times *out);
+
+/* search::periodicity on one host series (crates/search/src/lib.rs:33-145), 1:1. */
+int psr_periodicity(psr_ctx *ctx, const float *series, size_t n, double tsamp, double dm,
+                    const psr_search_cfg *cfg, psr_candidate *out, size_t cap, size_t *count);
+/* The spectral stages of periodicity for checks: raw |X|^2 and whitened power
+ * (crates/search/src/lib.rs:35-72), each nbins = next_pow2(n)+1 floats. */
+int psr_spectrum(psr_ctx *ctx, const float *series, size_t n, double tsamp,
+                 const psr_search_cfg *cfg, float *raw, float *white, size_t *nbins);
+
+/* ---- folding ------------------------------------------------------------ */
+/* fold::profile_at for `nprof` (period, sample_offset) pairs over one host series
+ * (crates/fold/src/lib.rs:16-80).  values/counts are [nprof*bins]. */
+int psr_fold(psr_ctx *ctx, const float *series, size_t n, double tsamp, const double *periods,
+             const uint64_t *offsets, size_t nprof, uint32_t bins, psr_profile_stats *stats,
+             float *values, uint64_t *counts);
+/* fold::refine (crates/fold/src/lib.rs:81-97): steps+1 trial frequencies, last max wins. */
+int psr_refine(psr_ctx *ctx, const float *series, size_t n, double tsamp, double frequency,
+               double span_hz, size_t steps, uint32_t bins, psr_profile_stats *stats,
+               float *values, uint64_t *counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,104 @@
+/*
+ * oracle.h -- CPU restatement of the PulsarSr search path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (pulsarsr_amd, libpsrgpu.so) never links or calls it.
+ *
+ * Every function restates one reference function line by line, in the
+ * reference's own arithmetic (f32 where the Rust uses f32, f64 where it uses
+ * f64, no fused multiply-add: build with -ffp-contract=off).  The one place
+ * that cannot be restated is the third-party FFT (realfft 3.5.0 -> rustfft
+ * 6.4.1, crates/search/src/lib.rs:41-43): the oracle computes the DFT in f64
+ * and rounds each component to f32, i.e. it models an exactly rounded f32
+ * transform.  Spectral parity is therefore checked to a tolerance, and the
+ * candidate lists are pinned against the saved reference reports.
+ *
+ * Functions without any independent pin are marked "parity unpinned" below;
+ * at present there are none (see the tests named next to each function).
+ */
+#ifndef PSR_ORACLE_H
+#define PSR_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORACLE_DISPERSION_SECONDS_MHZ 4148.808
+
+typedef struct {
+    double frequency_hz;
+    double period_s;
+    double dm;
+    uint64_t harmonics;
+    double score;
+} oracle_candidate;
+
+typedef struct {
+    double min_hz;
+    double max_hz;
+    double threshold;
+    uint64_t max_candidates;
+    const double *birdies; /* pairs (center_hz, half_width_hz) */
+    uint64_t nbirdies;
+    uint32_t max_harmonics; /* 16 = reference behaviour (h in 1,2,4,8,16) */
+    uint32_t pad_;
+} oracle_config;
+
+/* crates/dedisperse/src/lib.rs:9-34.  Returns 0, or -1 on invalid input. */
+int oracle_delays(const double *freqs, size_t nch, double tsamp, double dm, uint64_t *out);
+/* crates/dedisperse/src/lib.rs:35-44 */
+int oracle_dm_step(const double *freqs, size_t nch, double tsamp, double *out);
+/* crates/dedisperse/src/lib.rs:45-60.  Returns trial count (<= cap) or -1. */
+long oracle_plan(double min, double max, double step, double *out, size_t cap);
+/* crates/dedisperse/src/lib.rs:62-90.  length==0 means None.  *out_len set. */
+int oracle_apply(const float *data, size_t nsamples, size_t nch, const uint64_t *delays,
+                 size_t length, float *out, size_t *out_len);
+/* crates/fbank/src/lib.rs:149-196 (payload part): packed time-major -> f32 channel-major. */
+int oracle_decode(const uint8_t *payload, size_t payload_len, int nbits, size_t nchans,
+                  size_t limit, float *out, size_t *nsamples_out);
+/* crates/rfi/src/lib.rs:4-12 (upper median, total order) */
+float oracle_median(const float *v, size_t n);
+/* crates/rfi/src/lib.rs:31-96.  Mutates data; channels/blocks written as 0/1 flags. */
+double oracle_mask(float *data, size_t nsamples, size_t nch, size_t block_samples, float threshold,
+                   uint8_t *bad_channels, uint8_t *bad_blocks);
+
+/* f64 real FFT of x[0..N) (N even): X[k], k=0..N/2.  Exact-arithmetic model. */
+void oracle_rfft64(const double *x, size_t N, double *re, double *im);
+
+/* crates/search/src/lib.rs:35-72: the f32 spectrum stages.
+ * raw[N/2+1] = |X|^2 (f32), white[N/2+1] = whitened + DC/birdies.  Returns N_fft. */
+size_t oracle_spectrum(const float *series, size_t n, double tsamp, const oracle_config *cfg,
+                       float *raw, float *white);
+/* crates/search/src/lib.rs:33-145.  Returns the number of candidates written. */
+size_t oracle_periodicity(const float *series, size_t n, double tsamp, double dm,
+                          const oracle_config *cfg, oracle_candidate *out, size_t cap);
+/* All-f64 twin of oracle_periodicity (FP64 oracle for reduced-precision error reports). */
+size_t oracle_periodicity_f64(const float *series, size_t n, double tsamp, double dm,
+                              const oracle_config *cfg, oracle_candidate *out, size_t cap);
+/* crates/search/src/lib.rs:147-176 */
+size_t oracle_sift(const oracle_candidate *in, size_t n, double resolution_hz, size_t limit,
+                   int reject_dm_zero, oracle_candidate *out);
+
+/* crates/fold/src/lib.rs:16-80 */
+void oracle_profile_at(const float *series, size_t n, double tsamp, double period, size_t bins,
+                       size_t sample_offset, double *snr, double *chi2, float *values,
+                       uint64_t *counts);
+/* crates/fold/src/lib.rs:81-97.  Returns the winning period. */
+double oracle_refine(const float *series, size_t n, double tsamp, double frequency, double span_hz,
+                     size_t steps, size_t bins, double *snr, double *chi2, float *values,
+                     uint64_t *counts);
+
+/* Whole per-trial loop of crates/cli/src/main.rs:462-469 over f32 channel-major data,
+ * on `threads` POSIX threads (the CPU baseline).  counts[ntrials], out[ntrials*cap]. */
+int oracle_search_trials(const float *data, size_t nsamples, const double *freqs, size_t nch,
+                         double tsamp, const double *dms, size_t ntrials, size_t length,
+                         const oracle_config *cfg, oracle_candidate *out, size_t cap,
+                         uint32_t *counts, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

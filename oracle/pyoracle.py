@@ -1,0 +1,231 @@
+"""ctypes binding of oracle/liboracle.so (see oracle.h for each function's reference citation)."""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+class OCand(C.Structure):
+    _fields_ = [("frequency_hz", C.c_double), ("period_s", C.c_double), ("dm", C.c_double),
+                ("harmonics", C.c_uint64), ("score", C.c_double)]
+
+
+class OConfig(C.Structure):
+    _fields_ = [("min_hz", C.c_double), ("max_hz", C.c_double), ("threshold", C.c_double),
+                ("max_candidates", C.c_uint64), ("birdies", C.POINTER(C.c_double)), ("nbirdies", C.c_uint64),
+                ("max_harmonics", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp, fp, up = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint64)
+        sig = {
+            "oracle_delays": (C.c_int, [dp, C.c_size_t, C.c_double, C.c_double, up]),
+            "oracle_dm_step": (C.c_int, [dp, C.c_size_t, C.c_double, dp]),
+            "oracle_plan": (C.c_long, [C.c_double, C.c_double, C.c_double, dp, C.c_size_t]),
+            "oracle_apply": (C.c_int, [fp, C.c_size_t, C.c_size_t, up, C.c_size_t, fp, C.POINTER(C.c_size_t)]),
+            "oracle_decode": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_size_t, C.c_size_t, fp, C.POINTER(C.c_size_t)]),
+            "oracle_median": (C.c_float, [fp, C.c_size_t]),
+            "oracle_mask": (C.c_double, [fp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_void_p, C.c_void_p]),
+            "oracle_rfft64": (None, [dp, C.c_size_t, dp, dp]),
+            "oracle_spectrum": (C.c_size_t, [fp, C.c_size_t, C.c_double, C.POINTER(OConfig), fp, fp]),
+            "oracle_periodicity": (C.c_size_t, [fp, C.c_size_t, C.c_double, C.c_double, C.POINTER(OConfig), C.POINTER(OCand), C.c_size_t]),
+            "oracle_periodicity_f64": (C.c_size_t, [fp, C.c_size_t, C.c_double, C.c_double, C.POINTER(OConfig), C.POINTER(OCand), C.c_size_t]),
+            "oracle_sift": (C.c_size_t, [C.POINTER(OCand), C.c_size_t, C.c_double, C.c_size_t, C.c_int, C.POINTER(OCand)]),
+            "oracle_profile_at": (None, [fp, C.c_size_t, C.c_double, C.c_double, C.c_size_t, C.c_size_t, dp, dp, fp, up]),
+            "oracle_refine": (C.c_double, [fp, C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_size_t, dp, dp, fp, up]),
+            "oracle_search_trials": (C.c_int, [fp, C.c_size_t, dp, C.c_size_t, C.c_double, dp, C.c_size_t, C.c_size_t,
+                                               C.POINTER(OConfig), C.POINTER(OCand), C.c_size_t, C.POINTER(C.c_uint32), C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def delays(freqs, tsamp, dm):
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    out = np.zeros(len(f), dtype=np.uint64)
+    if lib().oracle_delays(_d(f), len(f), tsamp, dm, _u(out)):
+        raise ValueError("invalid dedispersion input")
+    return out
+
+
+def dm_step(freqs, tsamp):
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    out = C.c_double()
+    if lib().oracle_dm_step(_d(f), len(f), tsamp, C.byref(out)):
+        raise ValueError("invalid dedispersion input")
+    return out.value
+
+
+def plan(lo, hi, step):
+    cap = 100001
+    out = np.zeros(cap, dtype=np.float64)
+    n = lib().oracle_plan(lo, hi, step, _d(out), cap)
+    if n < 0:
+        raise ValueError("invalid DM bounds/step")
+    return out[:n]
+
+
+def apply(data, nsamples, dl, length=None):
+    d = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+    dl = np.ascontiguousarray(dl, dtype=np.uint64)
+    out = np.zeros(nsamples, dtype=np.float32)
+    n = C.c_size_t()
+    if lib().oracle_apply(_f(d), nsamples, len(dl), _u(dl), int(length or 0), _f(out), C.byref(n)):
+        raise ValueError("invalid dedispersion input")
+    return out[:n.value].copy()
+
+
+def decode(payload, nbits, nchans, limit=0):
+    p = np.ascontiguousarray(np.frombuffer(bytes(payload), dtype=np.uint8) if not isinstance(payload, np.ndarray) else payload)
+    total = p.nbytes * 8 // (nchans * nbits) if nchans * nbits else 0
+    ns_cap = min(limit, total) if limit else total
+    out = np.zeros(max(1, ns_cap * nchans), dtype=np.float32)
+    ns = C.c_size_t()
+    if lib().oracle_decode(p.ctypes.data, p.nbytes, nbits, nchans, int(limit), _f(out), C.byref(ns)):
+        raise ValueError("invalid or unsupported filterbank")
+    return out[:ns.value * nchans], ns.value
+
+
+def median(v):
+    a = np.ascontiguousarray(v, dtype=np.float32)
+    return float(lib().oracle_median(_f(a), len(a)))
+
+
+def mask(data, nsamples, block, threshold):
+    """rfi::mask in place on a float32 channel-major array; returns (channels, blocks, fraction)."""
+    assert data.dtype == np.float32 and data.flags.c_contiguous
+    nch = data.size // nsamples
+    bad = np.zeros(nch, dtype=np.uint8)
+    nblk = (nsamples + block - 1) // block
+    badb = np.zeros(nblk, dtype=np.uint8)
+    frac = lib().oracle_mask(_f(data), nsamples, nch, block, threshold, bad.ctypes.data, badb.ctypes.data)
+    return np.nonzero(bad)[0].tolist(), np.nonzero(badb)[0].tolist(), frac
+
+
+def _cfg(min_hz=1.0, max_hz=math.inf, threshold=6.0, max_candidates=32, birdies=(), max_harmonics=16):
+    b = np.ascontiguousarray(np.array(birdies, dtype=np.float64).reshape(-1))
+    c = OConfig(min_hz, max_hz, threshold, max_candidates, _d(b) if b.size else None, len(birdies), max_harmonics, 0)
+    return c, b
+
+
+def _cands(arr, n):
+    return [dict(frequency_hz=arr[i].frequency_hz, period_s=arr[i].period_s, dm=arr[i].dm,
+                 harmonics=int(arr[i].harmonics), score=arr[i].score) for i in range(n)]
+
+
+def periodicity(series, tsamp, dm, f64=False, **cfg):
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    c, keep = _cfg(**cfg)
+    cap = max(1, int(c.max_candidates))
+    out = (OCand * cap)()
+    fn = lib().oracle_periodicity_f64 if f64 else lib().oracle_periodicity
+    n = fn(_f(x), len(x), tsamp, dm, C.byref(c), out, cap)
+    del keep
+    return _cands(out, n)
+
+
+def spectrum(series, tsamp, **cfg):
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    size = 1
+    while size < len(x):
+        size <<= 1
+    nb = size + 1
+    raw = np.zeros(nb, dtype=np.float32)
+    white = np.zeros(nb, dtype=np.float32)
+    c, keep = _cfg(**cfg)
+    lib().oracle_spectrum(_f(x), len(x), tsamp, C.byref(c), _f(raw), _f(white))
+    del keep
+    return raw, white
+
+
+def rfft64(x):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    N = len(a)
+    re = np.zeros(N // 2 + 1)
+    im = np.zeros(N // 2 + 1)
+    lib().oracle_rfft64(_d(a), N, _d(re), _d(im))
+    return re + 1j * im
+
+
+def sift(cands, resolution_hz, limit, reject_dm_zero):
+    n = len(cands)
+    arr = (OCand * max(1, n))()
+    for i, c in enumerate(cands):
+        arr[i] = OCand(c["frequency_hz"], c["period_s"], c["dm"], c["harmonics"], c["score"])
+    out = (OCand * max(1, n))()
+    k = lib().oracle_sift(arr, n, resolution_hz, limit, int(bool(reject_dm_zero)), out)
+    return _cands(out, k)
+
+
+def profile_at(series, tsamp, period, bins, offset=0):
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    snr, chi = C.c_double(), C.c_double()
+    vals = np.zeros(bins, dtype=np.float32)
+    cnts = np.zeros(bins, dtype=np.uint64)
+    lib().oracle_profile_at(_f(x), len(x), tsamp, period, bins, offset, C.byref(snr), C.byref(chi), _f(vals), _u(cnts))
+    return dict(period_s=period, snr=snr.value, reduced_chi_square=chi.value, bins=vals, counts=cnts)
+
+
+def refine(series, tsamp, frequency, span_hz, steps, bins):
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    snr, chi = C.c_double(), C.c_double()
+    vals = np.zeros(bins, dtype=np.float32)
+    cnts = np.zeros(bins, dtype=np.uint64)
+    p = lib().oracle_refine(_f(x), len(x), tsamp, frequency, span_hz, steps, bins, C.byref(snr), C.byref(chi),
+                            _f(vals), _u(cnts))
+    return dict(period_s=p, snr=snr.value, reduced_chi_square=chi.value, bins=vals, counts=cnts)
+
+
+def search_trials(data, nsamples, freqs, tsamp, dms, length, threads=1, **cfg):
+    """The per-trial loop over f32 channel-major data -> list of candidate lists (trial order)."""
+    d = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    dm = np.ascontiguousarray(dms, dtype=np.float64)
+    c, keep = _cfg(**cfg)
+    cap = max(1, int(c.max_candidates))
+    out = (OCand * (cap * len(dm)))()
+    counts = np.zeros(len(dm), dtype=np.uint32)
+    rc = lib().oracle_search_trials(_f(d), nsamples, _d(f), len(f), tsamp, _d(dm), len(dm), length, C.byref(c),
+                                    out, cap, counts.ctypes.data_as(C.POINTER(C.c_uint32)), threads)
+    if rc:
+        raise ValueError("invalid dedispersion input")
+    del keep
+    res = []
+    for t in range(len(dm)):
+        res.append([dict(frequency_hz=out[t * cap + i].frequency_hz, period_s=out[t * cap + i].period_s,
+                         dm=out[t * cap + i].dm, harmonics=int(out[t * cap + i].harmonics), score=out[t * cap + i].score)
+                    for i in range(int(counts[t]))])
+    return res

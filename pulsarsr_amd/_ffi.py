@@ -1,0 +1,130 @@
+"""ctypes binding of libpsrgpu.so (include/psr_gpu.h) -- the C ABI of the search path.
+
+The library is built in-tree by ``pulsarsr_amd.build``.  Loading fails loudly if
+it is missing: there is no CPU fallback for the GPU entry points.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsrgpu.so")
+SYNTH_PATH = os.path.join(HERE, "libpsrsynth.so")
+
+PSR_OK = 0
+PSR_E_INVALID = -1
+PSR_E_SHAPE = -2
+PSR_E_DEVICE = -3
+PSR_E_OOM = -4
+PSR_E_FORMAT = -5
+PSR_E_CONTRACT = -6
+PSR_E_OVERFLOW = -7
+PRECISION_FP32 = 0
+PRECISION_FP16 = 1
+
+
+class Candidate(C.Structure):
+    """search::Candidate (crates/search/src/lib.rs:6-13)."""
+    _fields_ = [("frequency_hz", C.c_double), ("period_s", C.c_double), ("dm", C.c_double),
+                ("harmonics", C.c_uint64), ("score", C.c_double)]
+
+
+class SearchCfg(C.Structure):
+    _fields_ = [("min_hz", C.c_double), ("max_hz", C.c_double), ("threshold", C.c_double),
+                ("max_candidates", C.c_uint64), ("birdies", C.POINTER(C.c_double)),
+                ("nbirdies", C.c_uint64), ("max_harmonics", C.c_uint32), ("precision", C.c_uint32)]
+
+
+class Header(C.Structure):
+    _fields_ = [("nchans", C.c_uint32), ("nbits", C.c_uint32), ("tsamp", C.c_double),
+                ("fch1", C.c_double), ("foff", C.c_double), ("data_offset", C.c_uint64),
+                ("nsamples", C.c_uint64)]
+
+
+class Filterbank(C.Structure):
+    _fields_ = [("payload", C.c_void_p), ("payload_bytes", C.c_uint64), ("nbits", C.c_uint32),
+                ("nchans", C.c_uint32), ("nsamples", C.c_uint64), ("tsamp", C.c_double),
+                ("fch1", C.c_double), ("foff", C.c_double)]
+
+
+class ProfileStats(C.Structure):
+    _fields_ = [("period_s", C.c_double), ("snr", C.c_double), ("reduced_chi_square", C.c_double)]
+
+
+class StageTimes(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("delays_ms", C.c_double), ("dedisperse_ms", C.c_double),
+                ("fft_ms", C.c_double), ("whiten_ms", C.c_double), ("harmsum_ms", C.c_double),
+                ("finalize_ms", C.c_double), ("trials", C.c_uint64), ("series_len", C.c_uint64),
+                ("fft_len", C.c_uint64)]
+
+
+class SynthPulsar(C.Structure):
+    _fields_ = [("period_s", C.c_double), ("dm", C.c_double), ("duty", C.c_double),
+                ("amplitude", C.c_double), ("phase", C.c_double)]
+
+
+class SynthConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("nsamples", C.c_uint64), ("nchans", C.c_uint32),
+                ("nbits", C.c_uint32), ("tsamp", C.c_double), ("fch1", C.c_double),
+                ("foff", C.c_double), ("mean", C.c_double), ("sigma", C.c_double),
+                ("npulsars", C.c_uint32), ("nbursts", C.c_uint32),
+                ("pulsars", C.POINTER(SynthPulsar)), ("bursts", C.POINTER(C.c_double)),
+                ("sine_hz", C.c_double), ("sine_amp", C.c_double),
+                ("rfi_chans", C.POINTER(C.c_uint32)), ("nrfi", C.c_uint32), ("rfi_amp", C.c_double),
+                ("common", C.POINTER(C.c_float)), ("threads", C.c_uint32)]
+
+
+# (name, restype, argtypes) of every exported entry point of include/psr_gpu.h
+EXPORTS = [
+    ("psr_version", C.c_char_p, []),
+    ("psr_last_error", C.c_char_p, []),
+    ("psr_ctx_create", C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("psr_ctx_destroy", None, [C.c_void_p]),
+    ("psr_ctx_set_memory_cap", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("psr_delays", C.c_int, [C.POINTER(C.c_double), C.c_size_t, C.c_double, C.c_double, C.POINTER(C.c_uint64)]),
+    ("psr_dm_step", C.c_int, [C.POINTER(C.c_double), C.c_size_t, C.c_double, C.POINTER(C.c_double)]),
+    ("psr_plan", C.c_int, [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("psr_frequencies", C.c_int, [C.c_double, C.c_double, C.c_size_t, C.POINTER(C.c_double)]),
+    ("psr_parse_header", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(Header)]),
+    ("psr_sift", C.c_int, [C.POINTER(Candidate), C.c_size_t, C.c_double, C.c_size_t, C.c_int, C.POINTER(Candidate), C.POINTER(C.c_size_t)]),
+    ("psr_obs_create", C.c_int, [C.c_void_p, C.POINTER(Filterbank), C.POINTER(C.c_void_p)]),
+    ("psr_obs_create_f32", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
+    ("psr_obs_destroy", None, [C.c_void_p]),
+    ("psr_obs_download_f32", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("psr_dedisperse", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
+    ("psr_search", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t, C.POINTER(C.c_uint32)]),
+    ("psr_last_stage_times", C.c_int, [C.c_void_p, C.POINTER(StageTimes)]),
+    ("psr_periodicity", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("psr_spectrum", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(SearchCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
+    ("psr_fold", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
+    ("psr_refine", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
+]
+
+_lib = None
+_synth = None
+
+
+def lib():
+    """The loaded libpsrgpu.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libpsrgpu.so is not built (run pulsarsr_amd.build.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in EXPORTS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def synth_lib():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError("libpsrsynth.so is not built (run pulsarsr_amd.build.build())")
+        L = C.CDLL(SYNTH_PATH)
+        L.psr_synth_generate.restype = C.c_int
+        L.psr_synth_generate.argtypes = [C.POINTER(SynthConfig), C.c_void_p, C.c_uint64]
+        _synth = L
+    return _synth

@@ -1,0 +1,930 @@
+// C ABI of the MI355X search path (include/psr_gpu.h): context and device
+// memory management, observation ingest, and the orchestration of the
+// delays -> dedisperse -> FFT -> whiten -> harmonic sum -> rank chain that
+// replaces the reference's per-trial loop (crates/cli/src/main.rs:462-469).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/psr_gpu.h"
+#include "common.hpp"
+#include "host_internal.hpp"
+
+using namespace psr;
+
+namespace {
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? PSR_E_OOM : PSR_E_DEVICE,            \
+                        std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct psr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t mem_cap = 0;
+    uint64_t mem_used = 0;
+    DevBuf delays, dms, series, dd_partial, mean32, corr16, work, spec_t, raw_nat, scales,
+        fft_partial, peaks, peak_counts, cands, counts, twiddles, birdies, fold_series, fold_per,
+        fold_off, fold_sums, fold_counts;
+    SpecGeom tw_geom{};
+    bool tw_valid = false;
+    hipEvent_t ev[8] = {};
+    psr_stage_times last{};
+};
+
+struct psr_obs {
+    psr_ctx* ctx = nullptr;
+    int kind = KIND_U8;
+    uint32_t nchans = 0, nbits = 0, maxval = 0;
+    uint64_t nsamples = 0, pitch = 0;
+    double tsamp = 0, fch1 = 0, foff = 0;
+    void* data = nullptr;
+    double* inv_f2 = nullptr;  // device 1/(f*f) per channel
+    std::vector<double> freqs;
+    double ref_inv = 0;
+    double m0 = 0, s0 = 1;  // FP16 storage offset and scale
+};
+
+namespace {
+
+int ensure(psr_ctx* c, DevBuf& b, size_t bytes) {
+    if (b.bytes >= bytes) return PSR_OK;
+    if (b.p) {
+        (void)hipFree(b.p);
+        c->mem_used -= b.bytes;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    if (c->mem_cap && c->mem_used + bytes > c->mem_cap)
+        return fail(PSR_E_OOM, "device memory cap exceeded (" + std::to_string(c->mem_used + bytes) +
+                                   " > " + std::to_string(c->mem_cap) + " bytes)");
+    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(PSR_E_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    c->mem_used += bytes;
+    return PSR_OK;
+}
+
+void release(psr_ctx* c, DevBuf& b) {
+    if (b.p) {
+        (void)hipFree(b.p);
+        c->mem_used -= b.bytes;
+    }
+    b = DevBuf{};
+}
+
+int check_device(psr_ctx* c) {
+    HIP_TRY(hipSetDevice(c->device));
+    return PSR_OK;
+}
+
+// apply()'s length rules (crates/dedisperse/src/lib.rs:72-78) for the largest DM
+// of the batch; delays are non-decreasing in DM so it bounds every trial.
+int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t length,
+                   size_t* n_out, uint64_t* maxdelay_out) {
+    double dmax = 0.0;
+    for (size_t i = 0; i < ntrials; i++) {
+        if (!std::isfinite(dms[i]) || dms[i] < 0.0)
+            return fail(PSR_E_INVALID,
+                        "invalid dedispersion input: positive frequencies/tsamp and nonnegative DM required");
+        dmax = std::max(dmax, dms[i]);
+    }
+    std::vector<uint64_t> d(o->nchans);
+    int rc = dedisperse::delays(o->freqs.data(), o->nchans, o->tsamp, dmax, d.data());
+    if (rc) return rc;
+    const uint64_t maxd = *std::max_element(d.begin(), d.end());
+    if (maxd > o->nsamples) return fail(PSR_E_INVALID, "invalid dedispersion input: delay exceeds observation");
+    const uint64_t available = o->nsamples - maxd;
+    const uint64_t n = length ? length : available;
+    if (n < 2 || n > available) return fail(PSR_E_SHAPE, "invalid dedispersion input: invalid output length");
+    if (n > (1u << 23)) return fail(PSR_E_SHAPE, "series longer than 2^23 samples is not supported");
+    *n_out = (size_t)n;
+    *maxdelay_out = maxd;
+    return PSR_OK;
+}
+
+// Upper bound of the within-block delay spread of any channel: delays are
+// round(K*dm*a_c/tsamp) with a_c <= a_max, so the spread over a block of DMs is
+// at most K*a_max*(dm_hi - dm_lo)/tsamp + 1; two samples of slack on top.
+uint32_t spread_bound(const psr_obs* o, const double* dms_sorted, size_t ntrials, size_t block) {
+    double amax = 0.0;
+    for (double f : o->freqs) amax = std::max(amax, 1.0 / (f * f) - o->ref_inv);
+    double worst = 0.0;
+    for (size_t b = 0; b < ntrials; b += block) {
+        const size_t e = std::min(ntrials, b + block) - 1;
+        worst = std::max(worst, dms_sorted[e] - dms_sorted[b]);
+    }
+    return (uint32_t)std::ceil(DISPERSION_SECONDS_MHZ * amax * worst / o->tsamp) + 3u;
+}
+
+int upload_twiddles(psr_ctx* c, const SpecGeom& g) {
+    if (c->tw_valid && c->tw_geom.m == g.m && c->tw_geom.m1 == g.m1 && c->tw_geom.nfft == g.nfft)
+        return PSR_OK;
+    const size_t nt = twiddle_table_size(g);
+    std::vector<float2> t(nt);
+    fill_twiddles(g, t.data());
+    int rc = ensure(c, c->twiddles, nt * sizeof(float2));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(c->twiddles.p, t.data(), nt * sizeof(float2), hipMemcpyHostToDevice));
+    c->tw_geom = g;
+    c->tw_valid = true;
+    return PSR_OK;
+}
+
+// search::periodicity's bin limits (crates/search/src/lib.rs:73-80) and birdie
+// bins (lib.rs:62-72) for one geometry.
+int make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, SearchParams* p,
+                std::vector<uint32_t>* birdie_bins) {
+    std::memset(p, 0, sizeof(*p));
+    p->tsamp = tsamp;
+    p->bin_hz = 1.0 / ((double)g.nfft * tsamp);
+    p->threshold = cfg->threshold;
+    p->resolution = 1.0 / ((double)g.n * tsamp);
+    const double fq = std::ceil(cfg->min_hz / p->bin_hz);
+    const double firstd = std::max(fq, 1.0);
+    const double top = std::min(cfg->max_hz, 0.5 / tsamp);
+    const double lastd = std::floor(top / p->bin_hz);
+    const uint64_t first = firstd >= 4e9 ? 4000000000ull : (uint64_t)firstd;
+    uint64_t last = lastd >= 4e9 ? 4000000000ull : (lastd > 0 ? (uint64_t)lastd : 0);
+    last = std::min<uint64_t>(last, g.nbins - 2);
+    p->first = (uint32_t)std::min<uint64_t>(first, 4000000000ull);
+    p->last = (uint32_t)last;
+    const uint32_t maxh = cfg->max_harmonics ? cfg->max_harmonics : 16;
+    p->nlevels = 0;
+    for (int l = 0; l < 5; l++) {
+        const uint64_t h = 1ull << l;
+        if (h > maxh) break;
+        const uint64_t end = std::min<uint64_t>(last, (uint64_t)(g.nbins - 2) / h);
+        if (end <= first + 1) break;  // lib.rs:79-80 (later levels end earlier)
+        p->end_h[l] = (uint32_t)end;
+        p->nlevels = l + 1;
+    }
+    p->max_candidates = (uint32_t)std::min<uint64_t>(cfg->max_candidates, 1u << 30);
+    p->peak_cap = PEAK_CAP;
+    p->fp16 = cfg->precision == PSR_PRECISION_FP16;
+    birdie_bins->clear();
+    for (uint64_t b = 0; b < cfg->nbirdies; b++) {
+        const double cen = cfg->birdies[2 * b], w = cfg->birdies[2 * b + 1];
+        auto hit = [&](int64_t i) { return std::fabs((double)i * p->bin_hz - cen) <= w; };
+        const int64_t nb = g.nbins;
+        int64_t ic = (int64_t)std::floor(cen / p->bin_hz);
+        ic = std::min<int64_t>(std::max<int64_t>(ic, 0), nb - 1);
+        int64_t seed = -1;
+        for (int64_t i = std::max<int64_t>(ic - 1, 0); i <= std::min<int64_t>(ic + 2, nb - 1); i++)
+            if (hit(i)) {
+                seed = i;
+                break;
+            }
+        if (seed < 0) continue;
+        int64_t lo = 0, hi = seed;  // smallest hit in [0, seed]
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (hit(mid)) hi = mid;
+            else lo = mid + 1;
+        }
+        int64_t a = seed, z = nb - 1;  // largest hit in [seed, nb)
+        while (a < z) {
+            const int64_t mid = (a + z + 1) / 2;
+            if (hit(mid)) a = mid;
+            else z = mid - 1;
+        }
+        if (birdie_bins->size() / 2 >= (size_t)MAX_BIRDIE_INTERVALS)
+            return fail(PSR_E_INVALID, "more than 64 birdie intervals");
+        birdie_bins->push_back((uint32_t)lo);
+        birdie_bins->push_back((uint32_t)a);
+    }
+    p->nbirdie = (uint32_t)(birdie_bins->size() / 2);
+    return PSR_OK;
+}
+
+// Run the spectral chain (FFT, whitening, harmonic sums, ranking) over `nt`
+// series already in c->series (and their means/corrections), writing ranked
+// candidates to host `out`.  Events e1..e5 bracket the stages.
+int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
+                 const std::vector<uint32_t>& birdies, const double* dms_dev, uint32_t nt,
+                 psr_candidate* out, size_t cap_per_trial, uint32_t* counts, bool timed,
+                 uint32_t* overflow) {
+    const size_t es = p.fp16 ? 2 : 4;
+    const uint32_t npart = g.m1 == 1 ? 1u : g.m2 / 2 + 1;
+    int rc;
+    if ((rc = upload_twiddles(c, g))) return rc;
+    if ((rc = ensure(c, c->work, g.m1 == 1 ? 8 : (size_t)nt * g.m * sizeof(float2))) ||
+        (rc = ensure(c, c->spec_t, (size_t)nt * g.spec_pitch * es)) ||
+        (rc = ensure(c, c->raw_nat, (size_t)nt * g.spec_pitch * es)) ||
+        (rc = ensure(c, c->scales, (size_t)nt * g.nblocks * sizeof(float))) ||
+        (rc = ensure(c, c->fft_partial, (size_t)nt * npart * sizeof(double))) ||
+        (rc = ensure(c, c->peaks, (size_t)nt * PEAK_CAP * sizeof(RawPeak))) ||
+        (rc = ensure(c, c->peak_counts, (size_t)nt * sizeof(uint32_t))) ||
+        (rc = ensure(c, c->cands, (size_t)nt * cap_per_trial * sizeof(CandidateOut))) ||
+        (rc = ensure(c, c->counts, (size_t)(nt + 1) * sizeof(uint32_t))) ||
+        (rc = ensure(c, c->birdies, std::max<size_t>(birdies.size(), 1) * sizeof(uint32_t))))
+        return rc;
+    hipStream_t s = c->stream;
+    std::vector<uint32_t> bh(2 * MAX_BIRDIE_INTERVALS, 0);
+    std::copy(birdies.begin(), birdies.end(), bh.begin());
+    HIP_TRY(hipMemsetAsync(c->peak_counts.p, 0, (size_t)nt * sizeof(uint32_t), s));
+    HIP_TRY(hipMemsetAsync(c->counts.p, 0, (size_t)(nt + 1) * sizeof(uint32_t), s));
+    if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
+    launch_fft(s, g, c->series.p, (int)p.fp16, (const float*)c->mean32.p, (const float*)c->corr16.p,
+               nt, (float2*)c->work.p, c->spec_t.p, (double*)c->fft_partial.p,
+               (const float2*)c->twiddles.p);
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
+    launch_whiten(s, g, c->spec_t.p, (int)p.fp16, nt, (const double*)c->fft_partial.p, npart,
+                  c->raw_nat.p, (float*)c->scales.p);
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
+    launch_harmsum(s, g, p, c->raw_nat.p, (const float*)c->scales.p, bh.data(), nt,
+                   (RawPeak*)c->peaks.p, (uint32_t*)c->peak_counts.p);
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(c->ev[5], s));
+    launch_finalize(s, g, p, dms_dev, nt, (const RawPeak*)c->peaks.p,
+                    (const uint32_t*)c->peak_counts.p, (CandidateOut*)c->cands.p,
+                    (uint32_t)cap_per_trial, (uint32_t*)c->counts.p);
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(c->ev[6], s));
+    std::vector<uint32_t> cnt(nt + 1);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), c->counts.p, (size_t)(nt + 1) * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out, c->cands.p, (size_t)nt * cap_per_trial * sizeof(CandidateOut),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memcpy(counts, cnt.data(), nt * sizeof(uint32_t));
+    *overflow += cnt[nt];
+    return PSR_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+const char* psr_version(void) { return "pulsarsr_amd 0.1.0 (gfx950)"; }
+const char* psr_last_error(void) { return last_error(); }
+
+int psr_ctx_create(int device, uint32_t flags, psr_ctx** out) {
+    (void)flags;
+    clear_error();
+    if (!out) return fail(PSR_E_INVALID, "null output pointer");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return fail(PSR_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(PSR_E_DEVICE, "device index out of range");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PSR_E_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    psr_ctx* c = new psr_ctx();
+    c->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; e == hipSuccess && i < 8; i++) e = hipEventCreate(&c->ev[i]);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PSR_E_DEVICE, std::string("stream/event creation failed: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return PSR_OK;
+}
+
+void psr_ctx_destroy(psr_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf* all[] = {&c->delays, &c->dms, &c->series, &c->dd_partial, &c->mean32, &c->corr16,
+                     &c->work, &c->spec_t, &c->raw_nat, &c->scales, &c->fft_partial, &c->peaks,
+                     &c->peak_counts, &c->cands, &c->counts, &c->twiddles, &c->birdies,
+                     &c->fold_series, &c->fold_per, &c->fold_off, &c->fold_sums, &c->fold_counts};
+    for (DevBuf* b : all) release(c, *b);
+    for (int i = 0; i < 8; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int psr_ctx_set_memory_cap(psr_ctx* c, uint64_t bytes) {
+    if (!c) return fail(PSR_E_INVALID, "null context");
+    c->mem_cap = bytes;
+    return PSR_OK;
+}
+
+int psr_delays(const double* freqs, size_t nch, double tsamp, double dm, uint64_t* out) {
+    clear_error();
+    if ((!freqs && nch) || !out) return fail(PSR_E_INVALID, "null pointer");
+    return dedisperse::delays(freqs, nch, tsamp, dm, out);
+}
+int psr_dm_step(const double* freqs, size_t nch, double tsamp, double* out) {
+    clear_error();
+    if ((!freqs && nch) || !out) return fail(PSR_E_INVALID, "null pointer");
+    return dedisperse::dm_step(freqs, nch, tsamp, out);
+}
+int psr_plan(double min, double max, double step, double* out, size_t cap, size_t* count) {
+    clear_error();
+    if (!count) return fail(PSR_E_INVALID, "null pointer");
+    return dedisperse::plan(min, max, step, out, cap, count);
+}
+int psr_frequencies(double fch1, double foff, size_t nch, double* out) {
+    clear_error();
+    if (!out && nch) return fail(PSR_E_INVALID, "null pointer");
+    for (size_t c = 0; c < nch; c++) out[c] = fch1 + (double)c * foff;  // fbank/src/lib.rs:42
+    return PSR_OK;
+}
+int psr_parse_header(const uint8_t* bytes, size_t len, psr_header* out) {
+    clear_error();
+    if (!bytes || !out) return fail(PSR_E_INVALID, "null pointer");
+    return fbank::parse_header(bytes, len, out);
+}
+int psr_sift(const psr_candidate* in, size_t n, double resolution_hz, size_t limit,
+             int reject_dm_zero, psr_candidate* out, size_t* count) {
+    clear_error();
+    if ((!in && n) || !out || !count) return fail(PSR_E_INVALID, "null pointer");
+    return search::sift(in, n, resolution_hz, limit, reject_dm_zero, out, count);
+}
+
+// ------------------------------------------------------------ observations
+static int obs_finish(psr_ctx* c, psr_obs* o) {
+    // channel frequencies and 1/(f*f) (fbank/src/lib.rs:40-44; dedisperse/src/lib.rs:21-26)
+    o->freqs.resize(o->nchans);
+    for (uint32_t ch = 0; ch < o->nchans; ch++) o->freqs[ch] = o->fch1 + (double)ch * o->foff;
+    double fmax = 0.0;
+    for (double f : o->freqs) {
+        if (!std::isfinite(f) || f <= 0.0) return fail(PSR_E_FORMAT, "invalid or unsupported filterbank: nonpositive frequency");
+        fmax = std::fmax(fmax, f);
+    }
+    o->ref_inv = 1.0 / (fmax * fmax);
+    std::vector<double> inv(o->nchans);
+    for (uint32_t ch = 0; ch < o->nchans; ch++) inv[ch] = 1.0 / (o->freqs[ch] * o->freqs[ch]);
+    HIP_TRY(hipMalloc((void**)&o->inv_f2, o->nchans * sizeof(double)));
+    HIP_TRY(hipMemcpy(o->inv_f2, inv.data(), o->nchans * sizeof(double), hipMemcpyHostToDevice));
+    // FP16 storage offset/scale: sum of channel means (rounded to an integer for
+    // integer samples) and sqrt of the summed channel variances.
+    double *dsum = nullptr, *dsq = nullptr;
+    HIP_TRY(hipMalloc((void**)&dsum, o->nchans * sizeof(double)));
+    HIP_TRY(hipMalloc((void**)&dsq, o->nchans * sizeof(double)));
+    launch_chan_stats(c->stream, o->data, o->kind, o->nchans, o->nsamples, o->pitch, dsum, dsq);
+    std::vector<double> hs(o->nchans), hq(o->nchans);
+    HIP_TRY(hipMemcpyAsync(hs.data(), dsum, o->nchans * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hq.data(), dsq, o->nchans * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(dsum);
+    (void)hipFree(dsq);
+    double m = 0.0, v = 0.0;
+    for (uint32_t ch = 0; ch < o->nchans; ch++) {
+        const double mu = hs[ch] / (double)o->nsamples;
+        m += mu;
+        v += std::max(hq[ch] / (double)o->nsamples - mu * mu, 0.0);
+    }
+    o->m0 = o->kind == KIND_F32 ? m : std::round(m);
+    o->s0 = v > 1e-12 ? std::sqrt(v) : 1.0;
+    return PSR_OK;
+}
+
+int psr_obs_create(psr_ctx* c, const psr_filterbank* fb, psr_obs** out) {
+    clear_error();
+    if (!c || !fb || !out || !fb->payload) return fail(PSR_E_INVALID, "null pointer");
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    const uint32_t nb = fb->nbits;
+    if (!(nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16 || nb == 32) || fb->nchans == 0 ||
+        !(fb->tsamp > 0.0) || !std::isfinite(fb->tsamp))
+        return fail(PSR_E_FORMAT, "invalid or unsupported filterbank: invalid nchans/tsamp/nbits");
+    const uint64_t frame_bits = (uint64_t)fb->nchans * nb;
+    if ((fb->payload_bytes * 8) % frame_bits != 0)
+        return fail(PSR_E_FORMAT, "invalid or unsupported filterbank: truncated frame or packed padding unsupported");
+    const uint64_t total = fb->payload_bytes * 8 / frame_bits;
+    const uint64_t ns = fb->nsamples ? std::min(fb->nsamples, total) : total;
+    if (ns < 2) return fail(PSR_E_FORMAT, "invalid or unsupported filterbank: need at least two samples");
+    if (nb == 32) {  // fbank/src/lib.rs:185-187: reject non-finite samples
+        const float* f = (const float*)fb->payload;
+        for (uint64_t i = 0; i < ns * fb->nchans; i++)
+            if (!std::isfinite(f[i]))
+                return fail(PSR_E_FORMAT, "invalid or unsupported filterbank: nonfinite sample at " +
+                                              std::to_string(i / fb->nchans) + "/" +
+                                              std::to_string(i % fb->nchans));
+    }
+    psr_obs* o = new psr_obs();
+    o->ctx = c;
+    o->nchans = fb->nchans;
+    o->nbits = nb;
+    o->nsamples = ns;
+    o->tsamp = fb->tsamp;
+    o->fch1 = fb->fch1;
+    o->foff = fb->foff;
+    o->kind = nb <= 8 ? KIND_U8 : (nb == 16 ? KIND_U16 : KIND_F32);
+    o->maxval = nb <= 8 ? (1u << nb) - 1u : 0u;
+    const size_t es = o->kind == KIND_U8 ? 1 : (o->kind == KIND_U16 ? 2 : 4);
+    o->pitch = ((ns + 255) & ~(uint64_t)255) + 256;
+    const size_t bytes = (size_t)o->pitch * o->nchans * es;
+    const size_t pbytes = (size_t)((ns * frame_bits + 7) / 8);
+    void* dpay = nullptr;
+    hipError_t e = hipMalloc(&o->data, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dpay, pbytes + 16);
+    if (e == hipSuccess) e = hipMemsetAsync(o->data, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dpay, fb->payload, pbytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_ingest(c->stream, (const uint8_t*)dpay, (int)nb, o->nchans, ns, o->data, o->pitch, o->kind);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (dpay) (void)hipFree(dpay);
+    if (e != hipSuccess) {
+        psr_obs_destroy(o);
+        return fail(e == hipErrorOutOfMemory ? PSR_E_OOM : PSR_E_DEVICE,
+                    std::string("observation upload failed: ") + hipGetErrorString(e));
+    }
+    if ((rc = obs_finish(c, o))) {
+        psr_obs_destroy(o);
+        return rc;
+    }
+    *out = o;
+    return PSR_OK;
+}
+
+int psr_obs_create_f32(psr_ctx* c, const float* data, uint64_t nsamples, uint32_t nchans,
+                       double tsamp, double fch1, double foff, psr_obs** out) {
+    clear_error();
+    if (!c || !data || !out) return fail(PSR_E_INVALID, "null pointer");
+    if (nsamples == 0 || nchans == 0) return fail(PSR_E_SHAPE, "invalid dedispersion input: array shape mismatch");
+    if (!(tsamp > 0.0) || !std::isfinite(tsamp)) return fail(PSR_E_INVALID, "invalid dedispersion input: positive frequencies/tsamp and nonnegative DM required");
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    // Integer samples in [0, 255] sum exactly in any order: use the u8 layout.
+    const uint64_t total = nsamples * nchans;
+    bool small_int = true;
+    float vmax = 0.f;
+    for (uint64_t i = 0; i < total && small_int; i++) {
+        const float v = data[i];
+        small_int = v >= 0.f && v <= 255.f && v == std::floor(v);
+        vmax = std::max(vmax, v);
+    }
+    psr_obs* o = new psr_obs();
+    o->ctx = c;
+    o->nchans = nchans;
+    o->nbits = 32;
+    o->nsamples = nsamples;
+    o->tsamp = tsamp;
+    o->fch1 = fch1;
+    o->foff = foff;
+    o->kind = small_int ? KIND_U8 : KIND_F32;
+    o->maxval = small_int ? std::max<uint32_t>(1u, (uint32_t)vmax) : 0u;
+    const size_t es = small_int ? 1 : 4;
+    o->pitch = ((nsamples + 255) & ~(uint64_t)255) + 256;
+    const size_t bytes = (size_t)o->pitch * nchans * es;
+    hipError_t e = hipMalloc(&o->data, bytes);
+    if (e == hipSuccess) e = hipMemset(o->data, 0, bytes);
+    if (e == hipSuccess) {
+        if (small_int) {
+            std::vector<uint8_t> row(nsamples);
+            for (uint32_t ch = 0; ch < nchans && e == hipSuccess; ch++) {
+                for (uint64_t t = 0; t < nsamples; t++) row[t] = (uint8_t)data[ch * nsamples + t];
+                e = hipMemcpy((uint8_t*)o->data + ch * o->pitch, row.data(), nsamples, hipMemcpyHostToDevice);
+            }
+        } else {
+            e = hipMemcpy2D(o->data, o->pitch * 4, data, nsamples * 4, nsamples * 4, nchans,
+                            hipMemcpyHostToDevice);
+        }
+    }
+    if (e != hipSuccess) {
+        psr_obs_destroy(o);
+        return fail(e == hipErrorOutOfMemory ? PSR_E_OOM : PSR_E_DEVICE,
+                    std::string("observation upload failed: ") + hipGetErrorString(e));
+    }
+    if ((rc = obs_finish(c, o))) {
+        psr_obs_destroy(o);
+        return rc;
+    }
+    *out = o;
+    return PSR_OK;
+}
+
+void psr_obs_destroy(psr_obs* o) {
+    if (!o) return;
+    if (o->data) (void)hipFree(o->data);
+    if (o->inv_f2) (void)hipFree(o->inv_f2);
+    delete o;
+}
+
+int psr_obs_download_f32(psr_obs* o, float* out) {
+    clear_error();
+    if (!o || !out) return fail(PSR_E_INVALID, "null pointer");
+    const size_t es = o->kind == KIND_U8 ? 1 : (o->kind == KIND_U16 ? 2 : 4);
+    std::vector<uint8_t> row(o->nsamples * es);
+    for (uint32_t ch = 0; ch < o->nchans; ch++) {
+        HIP_TRY(hipMemcpy(row.data(), (uint8_t*)o->data + (size_t)ch * o->pitch * es, o->nsamples * es,
+                          hipMemcpyDeviceToHost));
+        for (uint64_t t = 0; t < o->nsamples; t++) {
+            float v;
+            if (o->kind == KIND_U8) v = row[t];
+            else if (o->kind == KIND_U16) v = ((uint16_t*)row.data())[t];
+            else v = ((float*)row.data())[t];
+            out[(size_t)ch * o->nsamples + t] = v;
+        }
+    }
+    return PSR_OK;
+}
+
+// ------------------------------------------------------------ dedispersion
+// Dedisperse `nt` trials (ascending DMs already on the device in c->dms) into
+// c->series at `g.series_pitch`, with per-trial means for the FFT input.
+static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const double* dms_host,
+                           uint32_t nt, int fp16, bool timed) {
+    hipStream_t s = c->stream;
+    const size_t es = fp16 ? 2 : 4;
+    const uint32_t n = g.n;
+    int rc;
+    const bool int_path = o->kind == KIND_U8;
+    const uint32_t ntiles = int_path ? (n + 511) / 512 : (n + 255) / 256;
+    if ((rc = ensure(c, c->delays, (size_t)nt * o->nchans * sizeof(int32_t))) ||
+        (rc = ensure(c, c->series, (size_t)nt * g.series_pitch * es + 64)) ||
+        (rc = ensure(c, c->dd_partial, (size_t)nt * ntiles * sizeof(double))) ||
+        (rc = ensure(c, c->mean32, (size_t)nt * sizeof(float))) ||
+        (rc = ensure(c, c->corr16, (size_t)nt * sizeof(float))))
+        return rc;
+    if (timed) HIP_TRY(hipEventRecord(c->ev[0], s));
+    launch_delays(s, o->inv_f2, o->nchans, o->ref_inv, o->tsamp, (const double*)c->dms.p, nt,
+                  (int32_t*)c->delays.p);
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
+    const double inv_s0 = 1.0 / o->s0;
+    if (int_path) {
+        const uint32_t spread = spread_bound(o, dms_host, nt, 32);
+        launch_dedisp_u8(s, (const uint8_t*)o->data, o->pitch, o->nchans, o->maxval,
+                         (const int32_t*)c->delays.p, nt, n, c->series.p, g.series_pitch, fp16,
+                         (float)o->m0, (float)inv_s0, (double*)c->dd_partial.p, spread);
+    } else {
+        launch_dedisp_f32(s, o->data, o->kind, o->pitch, o->nchans, (const int32_t*)c->delays.p, nt,
+                          n, c->series.p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
+                          (double*)c->dd_partial.p);
+    }
+    HIP_TRY(hipGetLastError());
+    launch_trial_mean(s, (const double*)c->dd_partial.p, ntiles, nt, n, o->m0, inv_s0,
+                      (float*)c->mean32.p, (float*)c->corr16.p);
+    HIP_TRY(hipGetLastError());
+    return PSR_OK;
+}
+
+int psr_dedisperse(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t length,
+                   float* out, size_t* out_len) {
+    clear_error();
+    if (!c || !o || (!dms && ntrials) || !out_len) return fail(PSR_E_INVALID, "null pointer");
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    if (ntrials == 0) {
+        *out_len = 0;
+        return PSR_OK;
+    }
+    size_t n;
+    uint64_t maxd;
+    if ((rc = resolve_length(o, dms, ntrials, length, &n, &maxd))) return rc;
+    *out_len = n;
+    if (!out) return PSR_OK;
+    std::vector<size_t> order(ntrials);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return dms[a] < dms[b]; });
+    std::vector<double> sorted(ntrials);
+    for (size_t i = 0; i < ntrials; i++) sorted[i] = dms[order[i]];
+    const SpecGeom g = make_geometry((uint32_t)n);
+    const size_t chunk = 1024;
+    std::vector<float> rowbuf;
+    for (size_t b = 0; b < ntrials; b += chunk) {
+        const uint32_t nt = (uint32_t)std::min(chunk, ntrials - b);
+        if ((rc = ensure(c, c->dms, nt * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpy(c->dms.p, sorted.data() + b, nt * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = dedisperse_into(c, o, g, sorted.data() + b, nt, 0, false))) return rc;
+        rowbuf.resize((size_t)nt * g.series_pitch);
+        HIP_TRY(hipMemcpyAsync(rowbuf.data(), c->series.p, rowbuf.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < nt; i++)
+            std::memcpy(out + order[b + i] * n, rowbuf.data() + (size_t)i * g.series_pitch, n * sizeof(float));
+    }
+    return PSR_OK;
+}
+
+// ------------------------------------------------------------------ search
+int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t length,
+               const psr_search_cfg* cfg, psr_candidate* out, size_t cap_per_trial,
+               uint32_t* counts) {
+    clear_error();
+    if (!c || !o || !cfg || (!dms && ntrials) || (!out && ntrials) || (!counts && ntrials))
+        return fail(PSR_E_INVALID, "null pointer");
+    if (cap_per_trial == 0) return fail(PSR_E_INVALID, "cap_per_trial must be positive");
+    if (cfg->precision != PSR_PRECISION_FP32 && cfg->precision != PSR_PRECISION_FP16)
+        return fail(PSR_E_INVALID, "unknown precision");
+    if (!(o->tsamp > 0.0) || !std::isfinite(o->tsamp))
+        return fail(PSR_E_CONTRACT, "periodicity requires finite positive tsamp");
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    c->last = psr_stage_times{};
+    if (ntrials == 0) return PSR_OK;
+    size_t n;
+    uint64_t maxd;
+    if ((rc = resolve_length(o, dms, ntrials, length, &n, &maxd))) return rc;
+    const SpecGeom g = make_geometry((uint32_t)n);
+    SearchParams p;
+    std::vector<uint32_t> birdies;
+    if ((rc = make_params(g, o->tsamp, cfg, &p, &birdies))) return rc;
+    std::vector<size_t> order(ntrials);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return dms[a] < dms[b]; });
+    std::vector<double> sorted(ntrials);
+    for (size_t i = 0; i < ntrials; i++) sorted[i] = dms[order[i]];
+    // trials per chunk: keep the chunk's working set near 16 GiB
+    const size_t es = p.fp16 ? 2 : 4;
+    const size_t per_trial = (size_t)g.series_pitch * es + (g.m1 == 1 ? 0 : (size_t)g.m * 8) +
+                             2 * (size_t)g.spec_pitch * es + (size_t)PEAK_CAP * sizeof(RawPeak) +
+                             cap_per_trial * sizeof(CandidateOut) + (size_t)o->nchans * 4 + 4096;
+    size_t chunk = std::max<size_t>(32, ((size_t)16 << 30) / per_trial);
+    chunk = std::min(chunk, ntrials);
+    chunk = std::max<size_t>(1, chunk);
+    std::vector<psr_candidate> hbuf(chunk * cap_per_trial);
+    std::vector<uint32_t> hcnt(chunk);
+    uint32_t overflow = 0;
+    double t_delay = 0, t_dd = 0, t_fft = 0, t_wh = 0, t_hs = 0, t_fin = 0;
+    for (size_t b = 0; b < ntrials; b += chunk) {
+        const uint32_t nt = (uint32_t)std::min(chunk, ntrials - b);
+        if ((rc = ensure(c, c->dms, nt * sizeof(double)))) return rc;
+        HIP_TRY(hipMemcpyAsync(c->dms.p, sorted.data() + b, nt * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+        if ((rc = dedisperse_into(c, o, g, sorted.data() + b, nt, (int)p.fp16, true))) return rc;
+        if ((rc = run_spectral(c, g, p, birdies, (const double*)c->dms.p, nt, hbuf.data(),
+                               cap_per_trial, hcnt.data(), true, &overflow)))
+            return rc;
+        t_delay += elapsed(c->ev[0], c->ev[1]);
+        t_dd += elapsed(c->ev[1], c->ev[2]);
+        t_fft += elapsed(c->ev[2], c->ev[3]);
+        t_wh += elapsed(c->ev[3], c->ev[4]);
+        t_hs += elapsed(c->ev[4], c->ev[5]);
+        t_fin += elapsed(c->ev[5], c->ev[6]);
+        for (uint32_t i = 0; i < nt; i++) {
+            const size_t dst = order[b + i];
+            counts[dst] = hcnt[i];
+            std::memcpy(out + dst * cap_per_trial, hbuf.data() + (size_t)i * cap_per_trial,
+                        hcnt[i] * sizeof(psr_candidate));
+        }
+    }
+    c->last.delays_ms = t_delay;
+    c->last.dedisperse_ms = t_dd;
+    c->last.fft_ms = t_fft;
+    c->last.whiten_ms = t_wh;
+    c->last.harmsum_ms = t_hs;
+    c->last.finalize_ms = t_fin;
+    c->last.total_ms = t_delay + t_dd + t_fft + t_wh + t_hs + t_fin;
+    c->last.trials = ntrials;
+    c->last.series_len = n;
+    c->last.fft_len = g.nfft;
+    if (overflow)
+        return fail(PSR_E_OVERFLOW, std::to_string(overflow) +
+                                        " trial(s) produced more peaks than the per-trial buffer holds");
+    return PSR_OK;
+}
+
+int psr_last_stage_times(const psr_ctx* c, psr_stage_times* out) {
+    if (!c || !out) return fail(PSR_E_INVALID, "null pointer");
+    *out = c->last;
+    return PSR_OK;
+}
+
+// Upload one host series as trial 0 of c->series with its f64 mean computed the
+// reference's way (sequential sum, crates/search/src/lib.rs:36).
+static int upload_series(psr_ctx* c, const float* series, size_t n, const SpecGeom& g) {
+    int rc;
+    if ((rc = ensure(c, c->series, (size_t)g.series_pitch * 4 + 64)) ||
+        (rc = ensure(c, c->mean32, sizeof(float))) || (rc = ensure(c, c->corr16, sizeof(float))) ||
+        (rc = ensure(c, c->dms, sizeof(double))))
+        return rc;
+    double mean = 0.0;
+    for (size_t i = 0; i < n; i++) mean += (double)series[i];
+    mean /= (double)n;
+    const float m32 = (float)mean, z = 0.f;
+    HIP_TRY(hipMemcpy(c->series.p, series, n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->mean32.p, &m32, sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->corr16.p, &z, sizeof(float), hipMemcpyHostToDevice));
+    return PSR_OK;
+}
+
+int psr_periodicity(psr_ctx* c, const float* series, size_t n, double tsamp, double dm,
+                    const psr_search_cfg* cfg, psr_candidate* out, size_t cap, size_t* count) {
+    clear_error();
+    if (!c || !series || !cfg || !out || !count) return fail(PSR_E_INVALID, "null pointer");
+    if (n < 2 || !std::isfinite(tsamp) || tsamp <= 0.0)  // lib.rs:34 assert!
+        return fail(PSR_E_CONTRACT, "periodicity requires at least two samples and finite positive tsamp");
+    if (n > (1u << 23)) return fail(PSR_E_SHAPE, "series longer than 2^23 samples is not supported");
+    if (cap == 0) return fail(PSR_E_INVALID, "cap must be positive");
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    const SpecGeom g = make_geometry((uint32_t)n);
+    psr_search_cfg cf = *cfg;
+    cf.precision = PSR_PRECISION_FP32;
+    SearchParams p;
+    std::vector<uint32_t> birdies;
+    if ((rc = make_params(g, tsamp, &cf, &p, &birdies))) return rc;
+    if ((rc = upload_series(c, series, n, g))) return rc;
+    HIP_TRY(hipMemcpy(c->dms.p, &dm, sizeof(double), hipMemcpyHostToDevice));
+    uint32_t cnt = 0, overflow = 0;
+    if ((rc = run_spectral(c, g, p, birdies, (const double*)c->dms.p, 1, out, cap, &cnt, false,
+                           &overflow)))
+        return rc;
+    *count = cnt;
+    if (overflow) return fail(PSR_E_OVERFLOW, "peak buffer overflow");
+    return PSR_OK;
+}
+
+int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp,
+                 const psr_search_cfg* cfg, float* raw, float* white, size_t* nbins) {
+    clear_error();
+    if (!c || !series || !cfg || !nbins) return fail(PSR_E_INVALID, "null pointer");
+    if (n < 2 || !std::isfinite(tsamp) || tsamp <= 0.0)
+        return fail(PSR_E_CONTRACT, "periodicity requires at least two samples and finite positive tsamp");
+    if (n > (1u << 23)) return fail(PSR_E_SHAPE, "series longer than 2^23 samples is not supported");
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    const SpecGeom g = make_geometry((uint32_t)n);
+    *nbins = g.nbins;
+    if (!raw && !white) return PSR_OK;
+    psr_search_cfg cf = *cfg;
+    cf.precision = PSR_PRECISION_FP32;
+    cf.threshold = INFINITY;  // no peaks needed
+    SearchParams p;
+    std::vector<uint32_t> birdies;
+    if ((rc = make_params(g, tsamp, &cf, &p, &birdies))) return rc;
+    if ((rc = upload_series(c, series, n, g))) return rc;
+    const double zero = 0.0;
+    HIP_TRY(hipMemcpy(c->dms.p, &zero, sizeof(double), hipMemcpyHostToDevice));
+    psr_candidate dummy[1];
+    uint32_t cnt = 0, overflow = 0;
+    if ((rc = run_spectral(c, g, p, birdies, (const double*)c->dms.p, 1, dummy, 1, &cnt, false,
+                           &overflow)))
+        return rc;
+    std::vector<float> r(g.nbins), sc(g.nblocks);
+    HIP_TRY(hipMemcpy(r.data(), c->raw_nat.p, g.nbins * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sc.data(), c->scales.p, g.nblocks * sizeof(float), hipMemcpyDeviceToHost));
+    if (raw) std::memcpy(raw, r.data(), g.nbins * sizeof(float));
+    if (white) {
+        for (uint32_t k = 0; k < g.nbins; k++) white[k] = r[k] / sc[k >> 6];
+        white[0] = 0.0f;
+        for (size_t b = 0; b + 1 < birdies.size(); b += 2)
+            for (uint32_t k = birdies[b]; k <= birdies[b + 1]; k++) white[k] = 0.0f;
+    }
+    return PSR_OK;
+}
+
+// ------------------------------------------------------------------- fold
+static inline int32_t total_key32(float f) {
+    int32_t b;
+    std::memcpy(&b, &f, 4);
+    b ^= (int32_t)(((uint32_t)(b >> 31)) >> 1);
+    return b;
+}
+
+// Profile statistics of crates/fold/src/lib.rs:32-79 from device bin sums.
+static void profile_stats(const double* sums, const uint64_t* cnt, uint32_t bins, double smean,
+                          double svar, double period, psr_profile_stats* st, float* values) {
+    for (uint32_t b = 0; b < bins; b++) values[b] = cnt[b] > 0 ? (float)(sums[b] / (double)cnt[b]) : 0.0f;
+    std::vector<float> v(values, values + bins);
+    std::vector<int32_t> keys(bins);
+    for (uint32_t b = 0; b < bins; b++) keys[b] = total_key32(values[b]);
+    std::nth_element(keys.begin(), keys.begin() + bins / 2, keys.end());
+    int32_t ck = keys[bins / 2];
+    ck ^= (int32_t)(((uint32_t)(ck >> 31)) >> 1);
+    float center;
+    std::memcpy(&center, &ck, 4);
+    std::vector<float> off;
+    for (uint32_t b = 0; b < bins; b++)
+        if (cnt[b] > 0) off.push_back(values[b]);
+    std::sort(off.begin(), off.end(), [](float a, float b) { return total_key32(a) < total_key32(b); });
+    const size_t keep = std::max<size_t>(off.size() * 3 / 4, 1);
+    off.resize(std::min(keep, off.size()));
+    double mean = 0.0;
+    for (float x : off) mean += (double)x;
+    mean /= (double)keep;
+    double ss = 0.0;
+    for (float x : off) {
+        const double d = (double)x - mean;
+        ss += d * d;
+    }
+    const double sigma = std::sqrt(ss / (double)keep);
+    float peak = -INFINITY;
+    for (uint32_t b = 0; b < bins; b++) peak = std::fmax(peak, values[b]);
+    st->snr = sigma > 1e-12 ? ((double)peak - (double)center) / sigma : 0.0;
+    uint32_t populated = 0;
+    for (uint32_t b = 0; b < bins; b++) populated += cnt[b] > 0;
+    double chi = 0.0;
+    if (svar > 1e-12 && populated > 1) {
+        for (uint32_t b = 0; b < bins; b++) {
+            const double d = (double)values[b] - smean;
+            chi += (double)cnt[b] * (d * d) / svar;
+        }
+        chi /= (double)(populated - 1);
+    }
+    st->reduced_chi_square = chi;
+    st->period_s = period;
+}
+
+int psr_fold(psr_ctx* c, const float* series, size_t n, double tsamp, const double* periods,
+             const uint64_t* offsets, size_t nprof, uint32_t bins, psr_profile_stats* stats,
+             float* values, uint64_t* counts) {
+    clear_error();
+    if (!c || !series || !periods || !stats || !values || !counts) return fail(PSR_E_INVALID, "null pointer");
+    if (bins < 4 || bins > 1024 || !(tsamp > 0.0) || !std::isfinite(tsamp))  // fold/src/lib.rs:23
+        return fail(PSR_E_CONTRACT, "fold requires 4 <= bins <= 1024 and finite positive tsamp");
+    for (size_t i = 0; i < nprof; i++)
+        if (!(periods[i] > 0.0) || !std::isfinite(periods[i]))
+            return fail(PSR_E_CONTRACT, "fold requires a finite positive period");
+    if (n == 0 || nprof == 0) return PSR_OK;
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    if ((rc = ensure(c, c->fold_series, n * sizeof(float))) ||
+        (rc = ensure(c, c->fold_per, nprof * sizeof(double))) ||
+        (rc = ensure(c, c->fold_off, nprof * sizeof(uint64_t))) ||
+        (rc = ensure(c, c->fold_sums, nprof * bins * sizeof(double))) ||
+        (rc = ensure(c, c->fold_counts, nprof * bins * sizeof(uint64_t))))
+        return rc;
+    std::vector<uint64_t> offs(nprof, 0);
+    if (offsets) std::copy(offsets, offsets + nprof, offs.begin());
+    HIP_TRY(hipMemcpy(c->fold_series.p, series, n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->fold_per.p, periods, nprof * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->fold_off.p, offs.data(), nprof * sizeof(uint64_t), hipMemcpyHostToDevice));
+    launch_fold(c->stream, (const float*)c->fold_series.p, n, tsamp, (const double*)c->fold_per.p,
+                (const uint64_t*)c->fold_off.p, (uint32_t)nprof, bins, (double*)c->fold_sums.p,
+                (uint64_t*)c->fold_counts.p);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> sums(nprof * bins);
+    HIP_TRY(hipMemcpyAsync(sums.data(), c->fold_sums.p, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(counts, c->fold_counts.p, nprof * bins * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    // series mean and variance, sequential f64 (fold/src/lib.rs:56-62)
+    double smean = 0.0;
+    for (size_t t = 0; t < n; t++) smean += (double)series[t];
+    smean /= (double)n;
+    double svar = 0.0;
+    for (size_t t = 0; t < n; t++) {
+        const double d = (double)series[t] - smean;
+        svar += d * d;
+    }
+    svar /= (double)n;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < nprof; i++)
+        profile_stats(sums.data() + i * bins, counts + i * bins, bins, smean, svar, periods[i],
+                      stats + i, values + i * bins);
+    return PSR_OK;
+}
+
+int psr_refine(psr_ctx* c, const float* series, size_t n, double tsamp, double frequency,
+               double span_hz, size_t steps, uint32_t bins, psr_profile_stats* stats,
+               float* values, uint64_t* counts) {
+    clear_error();
+    if (!(steps > 0 && frequency > span_hz && span_hz >= 0.0))  // fold/src/lib.rs:89
+        return fail(PSR_E_CONTRACT, "refine requires steps > 0 and frequency > span >= 0");
+    const size_t np = steps + 1;
+    std::vector<double> per(np);
+    for (size_t i = 0; i < np; i++) {
+        const double f = frequency - span_hz + 2.0 * span_hz * (double)i / (double)steps;
+        per[i] = 1.0 / f;
+    }
+    std::vector<psr_profile_stats> st(np);
+    std::vector<float> v((size_t)np * bins);
+    std::vector<uint64_t> cn((size_t)np * bins);
+    int rc = psr_fold(c, series, n, tsamp, per.data(), nullptr, np, bins, st.data(), v.data(), cn.data());
+    if (rc) return rc;
+    size_t best = 0;  // Iterator::max_by keeps the last maximum
+    for (size_t i = 1; i < np; i++) {
+        int64_t a, b;
+        double x = st[i].reduced_chi_square, y = st[best].reduced_chi_square;
+        std::memcpy(&a, &x, 8);
+        std::memcpy(&b, &y, 8);
+        a ^= (int64_t)(((uint64_t)(a >> 63)) >> 1);
+        b ^= (int64_t)(((uint64_t)(b >> 63)) >> 1);
+        if (a >= b) best = i;
+    }
+    *stats = st[best];
+    std::memcpy(values, v.data() + best * bins, bins * sizeof(float));
+    std::memcpy(counts, cn.data() + best * bins, bins * sizeof(uint64_t));
+    return PSR_OK;
+}
+
+}  // extern "C"

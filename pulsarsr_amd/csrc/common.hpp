@@ -1,0 +1,106 @@
+// Internal declarations shared by the host orchestration and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace psr {
+
+// Sample storage of a device-resident observation (channel-major rows).
+enum SampleKind : int { KIND_U8 = 0, KIND_U16 = 1, KIND_F32 = 2 };
+
+// Candidate record appended by the harmonic-sum kernel before ranking.
+struct RawPeak {
+    double score;
+    double offset;  // parabolic interpolation offset of the fundamental, [-0.5, 0.5]
+    uint32_t k;     // fundamental bin
+    uint32_t h;     // harmonics summed
+};
+
+// Output record of the ranking kernel; layout-identical to psr_candidate.
+struct CandidateOut {
+    double frequency_hz;
+    double period_s;
+    double dm;
+    uint64_t harmonics;
+    double score;
+};
+
+// Geometry of one periodicity search over a batch of equal-length series.
+struct SpecGeom {
+    uint32_t n;        // series length fed to periodicity
+    uint32_t nfft;     // 2 * next_pow2(n)
+    uint32_t m;        // nfft / 2 (complex FFT length)
+    uint32_t nbins;    // m + 1
+    uint32_t m1;       // second-pass FFT length (1 when single pass)
+    uint32_t m2;       // first-pass FFT length (m when single pass)
+    uint32_t nblocks;  // ceil(nbins / 64)
+    uint32_t series_pitch;  // elements between consecutive series rows
+    uint64_t spec_pitch;    // elements between consecutive spectrum rows
+};
+
+struct SearchParams {
+    double tsamp;
+    double bin_hz;
+    double threshold;
+    double resolution;  // 1 / (n * tsamp): per-trial dedup width
+    uint32_t first;
+    uint32_t last;
+    uint32_t end_h[5];  // end bin per harmonic level (0 = level skipped)
+    uint32_t nlevels;   // number of harmonic levels searched (h = 1,2,4,8,16)
+    uint32_t max_candidates;
+    uint32_t nbirdie;   // number of excluded bin intervals
+    uint32_t fp16;      // series/spectra stored FP16
+    uint32_t peak_cap;  // raw peak capacity per trial
+};
+
+constexpr int MAX_BIRDIE_INTERVALS = 64;
+constexpr uint32_t PEAK_CAP = 4096;
+
+// ---- kernel launchers (defined in the .hip files) ----
+// ingest: packed time-major payload -> channel-major rows
+void launch_ingest(hipStream_t s, const uint8_t* payload, int nbits, uint32_t nchans,
+                   uint64_t nsamples, void* out, uint64_t pitch, int kind);
+void launch_chan_stats(hipStream_t s, const void* data, int kind, uint32_t nchans,
+                       uint64_t nsamples, uint64_t pitch, double* sums, double* sumsq);
+void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double ref_inv,
+                   double tsamp, const double* dms, uint32_t ntrials, int32_t* out);
+// integer dedispersion (u8 rows), outputs f32 exact sums or FP16 scaled
+void launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
+                      uint32_t maxval, const int32_t* delays, uint32_t ntrials, uint32_t n,
+                      void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                      double* partial, uint32_t spread_max);
+// sequential-order f32 dedispersion (16-bit and float rows)
+void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch,
+                       uint32_t nchans, const int32_t* delays, uint32_t ntrials, uint32_t n,
+                       void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                       double* partial);
+// FFT + power
+void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
+                const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
+                void* spec_t, double* partial_sums, const float2* tw);
+// whitening scales + natural-order raw spectrum
+void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp16,
+                   uint32_t ntrials, const double* partial_sums, uint32_t npartials,
+                   void* raw_nat, float* scales);
+// harmonic sums + peaks
+void launch_harmsum(hipStream_t s, const SpecGeom& g, const SearchParams& p, const void* raw_nat,
+                    const float* scales, const uint32_t* birdies, uint32_t ntrials,
+                    RawPeak* peaks, uint32_t* peak_counts);
+// rank, dedup, cap
+void launch_finalize(hipStream_t s, const SpecGeom& g, const SearchParams& p, const double* dms,
+                     uint32_t ntrials, const RawPeak* peaks, const uint32_t* peak_counts,
+                     CandidateOut* out, uint32_t cap_per_trial, uint32_t* counts);
+void launch_trial_mean(hipStream_t s, const double* partial, uint32_t ntiles, uint32_t ntrials,
+                       uint32_t n, double m0, double inv_s0, float* mean32, float* corr16);
+// folding
+void launch_fold(hipStream_t s, const float* series, uint64_t n, double tsamp,
+                 const double* periods, const uint64_t* offsets, uint32_t nprof, uint32_t bins,
+                 double* sums, uint64_t* counts);
+
+// Twiddle tables needed by launch_fft for a geometry (size in float2 elements).
+size_t twiddle_table_size(const SpecGeom& g);
+void fill_twiddles(const SpecGeom& g, float2* host_table);
+SpecGeom make_geometry(uint32_t n);
+
+}  // namespace psr

@@ -1,0 +1,331 @@
+// Ingest, channel statistics, delay table and incoherent dedispersion kernels
+// for gfx950.  Dedispersion restates crates/dedisperse/src/lib.rs:62-90:
+//   out[t] = sum_c data[c][t + delay_c]
+// For samples of at most 8 bits the channel sums are integers below 2^24, so
+// they are exact (bit-identical to the reference's channel-ordered f32 sums) in
+// any order; the kernel accumulates them with packed 16-bit adds and widens to
+// 32 bits before any half-word can overflow.  16-bit and float samples use a
+// kernel that adds in channel order in f32, exactly as the reference does.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace psr {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// ---------------------------------------------------------------- ingest
+// Packed time-major SIGPROC frames -> channel-major rows (u8 for <=8 bits,
+// u16 for 16 bits, f32 for 32 bits).  Sample i = t*nchans + ch occupies bits
+// [i*nbits, (i+1)*nbits), LSB-first within a byte (crates/fbank/src/lib.rs:175-182).
+// One workgroup transposes a 64-frame x 64-channel tile through LDS.
+__global__ __launch_bounds__(256) void k_ingest(const uint8_t* __restrict__ payload, int nbits,
+                                                uint32_t nchans, uint64_t nsamples,
+                                                void* __restrict__ out, uint64_t pitch, int kind) {
+    __shared__ uint32_t tile[64][65];
+    const uint64_t t0 = (uint64_t)blockIdx.x * 64;
+    const uint32_t c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // read: lane tx walks channels of frame t0 + r
+    for (int r = ty; r < 64; r += 4) {
+        const uint64_t t = t0 + r;
+        const uint32_t c = c0 + tx;
+        uint32_t v = 0;
+        if (t < nsamples && c < nchans) {
+            const uint64_t i = t * nchans + c;
+            if (nbits <= 8) {
+                const uint64_t bit = i * (uint64_t)nbits;
+                v = (payload[bit >> 3] >> (bit & 7)) & ((1u << nbits) - 1u);
+            } else if (nbits == 16) {
+                v = (uint32_t)payload[2 * i] | ((uint32_t)payload[2 * i + 1] << 8);
+            } else {
+                v = (uint32_t)payload[4 * i] | ((uint32_t)payload[4 * i + 1] << 8) |
+                    ((uint32_t)payload[4 * i + 2] << 16) | ((uint32_t)payload[4 * i + 3] << 24);
+            }
+        }
+        tile[r][tx] = v;
+    }
+    __syncthreads();
+    // write: lane tx walks time of channel c0 + r
+    for (int r = ty; r < 64; r += 4) {
+        const uint32_t c = c0 + r;
+        const uint64_t t = t0 + tx;
+        if (c >= nchans || t >= nsamples) continue;
+        const uint32_t v = tile[tx][r];
+        if (kind == KIND_U8)
+            ((uint8_t*)out)[c * pitch + t] = (uint8_t)v;
+        else if (kind == KIND_U16)
+            ((uint16_t*)out)[c * pitch + t] = (uint16_t)v;
+        else
+            ((uint32_t*)out)[c * pitch + t] = v;
+    }
+}
+
+void launch_ingest(hipStream_t s, const uint8_t* payload, int nbits, uint32_t nchans,
+                   uint64_t nsamples, void* out, uint64_t pitch, int kind) {
+    dim3 grid((unsigned)((nsamples + 63) / 64), (nchans + 63) / 64);
+    hipLaunchKernelGGL(k_ingest, grid, dim3(256), 0, s, payload, nbits, nchans, nsamples, out,
+                       pitch, kind);
+}
+
+// ------------------------------------------------------- channel statistics
+// Per-channel sum and sum of squares in f64, reduced in a fixed tree order so
+// the result is deterministic.  Used only to choose the FP16 storage offset
+// and scale; it does not enter any exact result.
+__global__ __launch_bounds__(256) void k_chan_stats(const void* __restrict__ data, int kind,
+                                                    uint64_t nsamples, uint64_t pitch,
+                                                    double* __restrict__ sums,
+                                                    double* __restrict__ sumsq) {
+    const uint32_t c = blockIdx.x;
+    double s = 0.0, q = 0.0;
+    for (uint64_t t = threadIdx.x; t < nsamples; t += blockDim.x) {
+        double v;
+        if (kind == KIND_U8)
+            v = ((const uint8_t*)data)[c * pitch + t];
+        else if (kind == KIND_U16)
+            v = ((const uint16_t*)data)[c * pitch + t];
+        else
+            v = ((const float*)data)[c * pitch + t];
+        s += v;
+        q += v * v;
+    }
+    __shared__ double rs[256], rq[256];
+    rs[threadIdx.x] = s;
+    rq[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            rs[threadIdx.x] += rs[threadIdx.x + w];
+            rq[threadIdx.x] += rq[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[c] = rs[0];
+        sumsq[c] = rq[0];
+    }
+}
+
+void launch_chan_stats(hipStream_t s, const void* data, int kind, uint32_t nchans,
+                       uint64_t nsamples, uint64_t pitch, double* sums, double* sumsq) {
+    hipLaunchKernelGGL(k_chan_stats, dim3(nchans), dim3(256), 0, s, data, kind, nsamples, pitch,
+                       sums, sumsq);
+}
+
+// ------------------------------------------------------------ delay table
+// crates/dedisperse/src/lib.rs:21-33 in f64 without contraction:
+//   round(((4148.808 * dm) * (1/f^2 - 1/fmax^2)) / tsamp), clamped at 0.
+// inv_f2[c] = 1/(f*f) and ref_inv = 1/(fmax*fmax) are computed on the host the
+// same way, so the table is bit-identical to the host's delays().
+__global__ __launch_bounds__(256) void k_delays(const double* __restrict__ inv_f2,
+                                                uint32_t nchans, double ref_inv, double tsamp,
+                                                const double* __restrict__ dms, uint32_t ntrials,
+                                                int32_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)nchans * ntrials) return;
+    const uint32_t d = (uint32_t)(i / nchans), c = (uint32_t)(i % nchans);
+    const double k_dm = 4148.808 * dms[d];
+    const double samples = round(k_dm * (inv_f2[c] - ref_inv) / tsamp);
+    out[i] = (int32_t)fmax(samples, 0.0);
+}
+
+void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double ref_inv,
+                   double tsamp, const double* dms, uint32_t ntrials, int32_t* out) {
+    const uint64_t total = (uint64_t)nchans * ntrials;
+    hipLaunchKernelGGL(k_delays, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, inv_f2,
+                       nchans, ref_inv, tsamp, dms, ntrials, out);
+}
+
+// ------------------------------------------------- integer dedispersion (u8)
+// Workgroup = 4 waves sharing one time tile of DD_TILE outputs and a block of
+// DD_TRIALS consecutive DM trials (each wave owns DD_M trials).  Channels are
+// staged DD chunk by chunk into LDS as 32-bit words
+//     w[i] = s[base + i] | s[base + i + 64] << 16,
+// so one aligned ds_read_b32 at any delay offset yields the samples of two
+// outputs (t and t+64) of one lane, accumulated with a single v_pk_add_u16.
+// Lane l of a wave owns outputs t0 + l + 64*q, q = 0..2*DD_G-1.
+constexpr int DD_M = 8;
+constexpr int DD_G = 4;
+constexpr int DD_TILE = 128 * DD_G;
+constexpr int DD_TRIALS = 4 * DD_M;
+
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+    u16x2 x = __builtin_bit_cast(u16x2, a), y = __builtin_bit_cast(u16x2, b);
+    return __builtin_bit_cast(uint32_t, x + y);
+}
+
+__global__ __launch_bounds__(256) void k_dedisp_u8(
+    const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_chans,
+    const int32_t* __restrict__ delays, uint32_t ntrials, uint32_t n, void* __restrict__ out,
+    uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial,
+    uint32_t W, uint32_t C) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* rows = lds;                            // [C][W]
+    int32_t* dl = (int32_t*)(lds + (size_t)C * W);   // [C][DD_TRIALS]
+    const uint32_t tile = blockIdx.x;
+    const uint32_t t0 = tile * DD_TILE;
+    const uint32_t d0 = blockIdx.y * DD_TRIALS;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+
+    uint32_t acc16[DD_M][DD_G];
+    uint32_t acc32[DD_M][2 * DD_G];
+#pragma unroll
+    for (int m = 0; m < DD_M; m++) {
+#pragma unroll
+        for (int p = 0; p < DD_G; p++) acc16[m][p] = 0;
+#pragma unroll
+        for (int q = 0; q < 2 * DD_G; q++) acc32[m][q] = 0;
+    }
+    uint32_t in_phase = 0;
+    const uint32_t quads = W / 4;
+    for (uint32_t cc = 0; cc < nchans; cc += C) {
+        const uint32_t nc = min(C, nchans - cc);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nc * DD_TRIALS; i += blockDim.x) {
+            const uint32_t c = i / DD_TRIALS, tt = i % DD_TRIALS;
+            const uint32_t d = min(d0 + tt, ntrials - 1);
+            dl[c * DD_TRIALS + tt] = delays[(size_t)d * nchans + cc + c];
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nc * quads; i += blockDim.x) {
+            const uint32_t c = i / quads, q = i % quads;
+            const uint32_t base = (uint32_t)dl[c * DD_TRIALS] & ~3u;
+            const uint8_t* row = data + (size_t)(cc + c) * pitch;
+            const uint64_t off = (uint64_t)t0 + base + 4u * q;
+            const uint32_t a = off + 4 <= pitch ? *(const uint32_t*)(row + off) : 0u;
+            const uint32_t b = off + 68 <= pitch ? *(const uint32_t*)(row + off + 64) : 0u;
+            uint4 w;
+            w.x = __builtin_amdgcn_perm(b, a, 0x0c040c00u);
+            w.y = __builtin_amdgcn_perm(b, a, 0x0c050c01u);
+            w.z = __builtin_amdgcn_perm(b, a, 0x0c060c02u);
+            w.w = __builtin_amdgcn_perm(b, a, 0x0c070c03u);
+            *(uint4*)(rows + (size_t)c * W + 4 * q) = w;
+        }
+        __syncthreads();
+        for (uint32_t c = 0; c < nc; c++) {
+            const int32_t* dc = dl + c * DD_TRIALS;
+            const int32_t base = dc[0] & ~3;
+            const uint32_t* rowc = rows + (size_t)c * W + lane;
+#pragma unroll
+            for (int m = 0; m < DD_M; m++) {
+                const int e = __builtin_amdgcn_readfirstlane(dc[wave * DD_M + m] - base);
+                const uint32_t* r = rowc + e;
+#pragma unroll
+                for (int p = 0; p < DD_G; p++) acc16[m][p] = pk_add(acc16[m][p], r[128 * p]);
+            }
+            if (++in_phase == phase_chans) {
+                in_phase = 0;
+#pragma unroll
+                for (int m = 0; m < DD_M; m++)
+#pragma unroll
+                    for (int p = 0; p < DD_G; p++) {
+                        acc32[m][2 * p] += acc16[m][p] & 0xffffu;
+                        acc32[m][2 * p + 1] += acc16[m][p] >> 16;
+                        acc16[m][p] = 0;
+                    }
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < DD_M; m++)
+#pragma unroll
+        for (int p = 0; p < DD_G; p++) {
+            acc32[m][2 * p] += acc16[m][p] & 0xffffu;
+            acc32[m][2 * p + 1] += acc16[m][p] >> 16;
+        }
+    const uint32_t ntiles = gridDim.x;
+    const int m0i = (int)m0;
+#pragma unroll
+    for (int m = 0; m < DD_M; m++) {
+        const uint32_t d = d0 + wave * DD_M + m;
+        double lsum = 0.0;
+        if (d < ntrials) {
+#pragma unroll
+            for (int q = 0; q < 2 * DD_G; q++) {
+                const uint32_t t = t0 + lane + 64 * q;
+                const uint32_t v = acc32[m][(q >> 1) * 2 + (q & 1)];
+                if (t < n) {
+                    lsum += (double)v;
+                    if (fp16)
+                        ((__half*)out)[(size_t)d * out_pitch + t] =
+                            __float2half_rn((float)((int)v - m0i) * inv_s0);
+                    else
+                        ((float*)out)[(size_t)d * out_pitch + t] = (float)v;
+                }
+            }
+        }
+        // wave reduction of the tile's sum (integers < 2^53: exact in any order)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+        if (lane == 0 && d < ntrials) partial[(size_t)d * ntiles + tile] = lsum;
+    }
+}
+
+void launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
+                      uint32_t maxval, const int32_t* delays, uint32_t ntrials, uint32_t n,
+                      void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                      double* partial, uint32_t spread_max) {
+    // rows must cover the largest in-block delay spread, 3 bytes of alignment
+    // slack, and the lane/pair footprint of the tile.
+    uint32_t W = spread_max + 3 + 64 + 128 * (DD_G - 1) + 1;
+    W = (W + 3) & ~3u;
+    const uint32_t budget = 48 * 1024;
+    uint32_t C = 64;
+    while (C > 1 && (size_t)C * (W * 4 + DD_TRIALS * 4) > budget) C >>= 1;
+    uint32_t phase = 65535u / (maxval ? maxval : 1);
+    phase = phase > 4096 ? 4096 : phase;
+    // phases are whole numbers of chunks
+    phase = (phase / C) * C;
+    if (phase == 0) phase = 1;
+    const size_t shm = (size_t)C * (W * 4 + DD_TRIALS * 4);
+    dim3 grid((n + DD_TILE - 1) / DD_TILE, (ntrials + DD_TRIALS - 1) / DD_TRIALS);
+    hipLaunchKernelGGL(k_dedisp_u8, grid, dim3(256), shm, s, data, pitch, nchans, phase, delays,
+                       ntrials, n, out, out_pitch, fp16, m0, inv_s0, partial, W, C);
+}
+
+// ---------------------------------------- channel-ordered f32 dedispersion
+// For 16-bit and float samples: one thread per output, f32 adds in channel
+// order starting from 0.0 -- the reference's exact operation sequence.
+__global__ __launch_bounds__(256) void k_dedisp_f32(const void* __restrict__ data, int kind,
+                                                    uint64_t pitch, uint32_t nchans,
+                                                    const int32_t* __restrict__ delays,
+                                                    uint32_t ntrials, uint32_t n,
+                                                    void* __restrict__ out, uint32_t out_pitch,
+                                                    int fp16, float m0, float inv_s0,
+                                                    double* __restrict__ partial) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    const int32_t* dd = delays + (size_t)d * nchans;
+    float acc = 0.0f;
+    if (t < n) {
+        for (uint32_t c = 0; c < nchans; c++) {
+            const uint64_t idx = (uint64_t)c * pitch + t + (uint32_t)dd[c];
+            const float v = kind == KIND_U16 ? (float)((const uint16_t*)data)[idx]
+                                             : ((const float*)data)[idx];
+            acc += v;
+        }
+        if (fp16)
+            ((__half*)out)[(size_t)d * out_pitch + t] = __float2half_rn((acc - m0) * inv_s0);
+        else
+            ((float*)out)[(size_t)d * out_pitch + t] = acc;
+    }
+    double lsum = t < n ? (double)acc : 0.0;
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(size_t)d * gridDim.x + blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch,
+                       uint32_t nchans, const int32_t* delays, uint32_t ntrials, uint32_t n,
+                       void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                       double* partial) {
+    dim3 grid((n + 255) / 256, ntrials);
+    hipLaunchKernelGGL(k_dedisp_f32, grid, dim3(256), 0, s, data, kind, pitch, nchans, delays,
+                       ntrials, n, out, out_pitch, fp16, m0, inv_s0, partial);
+}
+
+}  // namespace psr

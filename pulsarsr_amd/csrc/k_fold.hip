@@ -1,0 +1,78 @@
+// Phase folding for gfx950 (crates/fold/src/lib.rs:25-31) and the per-trial
+// mean used by the FFT input (crates/search/src/lib.rs:36).
+//
+// Fold: one workgroup per (period, sample_offset) profile.  The bin of sample t
+// is computed in f64 exactly as the reference does,
+//   x = ((t + offset) * tsamp) / period;  phase = x - trunc(x);
+//   b = min((phase * bins) as usize, bins - 1),
+// so bin assignment is bit-identical.  Bin sums are f64 and counts u64; sums
+// of integer-valued series are exact in any order.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace psr {
+
+constexpr uint32_t FOLD_MAX_BINS = 1024;
+
+__global__ __launch_bounds__(256) void k_fold(const float* __restrict__ series, uint64_t n,
+                                              double tsamp, const double* __restrict__ periods,
+                                              const uint64_t* __restrict__ offsets, uint32_t bins,
+                                              double* __restrict__ sums,
+                                              uint64_t* __restrict__ counts) {
+    __shared__ double s_sum[FOLD_MAX_BINS];
+    __shared__ unsigned long long s_cnt[FOLD_MAX_BINS];
+    const uint32_t prof = blockIdx.x;
+    const double period = periods[prof];
+    const uint64_t off = offsets[prof];
+    for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x) {
+        s_sum[b] = 0.0;
+        s_cnt[b] = 0ull;
+    }
+    __syncthreads();
+    const double fb = (double)bins;
+    for (uint64_t t = threadIdx.x; t < n; t += blockDim.x) {
+        const double x = (double)(t + off) * tsamp / period;
+        const double phase = x - trunc(x);
+        uint64_t b = (uint64_t)(phase * fb);
+        if (b > bins - 1) b = bins - 1;
+        atomicAdd(&s_sum[b], (double)series[t]);
+        atomicAdd(&s_cnt[b], 1ull);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < bins; b += blockDim.x) {
+        sums[(size_t)prof * bins + b] = s_sum[b];
+        counts[(size_t)prof * bins + b] = s_cnt[b];
+    }
+}
+
+void launch_fold(hipStream_t s, const float* series, uint64_t n, double tsamp,
+                 const double* periods, const uint64_t* offsets, uint32_t nprof, uint32_t bins,
+                 double* sums, uint64_t* counts) {
+    hipLaunchKernelGGL(k_fold, dim3(nprof), dim3(256), 0, s, series, n, tsamp, periods, offsets,
+                       bins, sums, counts);
+}
+
+// mean32[t] = (sum/n) as f32 (FP32 path); corr16[t] = (m0 - sum/n) * inv_s0
+// (FP16 path: the offset that turns stored (x - m0)/s0 into (x - mean)/s0).
+// The trial sum is reduced from the dedispersion kernel's per-tile partials in
+// tile order.
+__global__ void k_trial_mean(const double* __restrict__ partial, uint32_t ntiles,
+                             uint32_t ntrials, uint32_t n, double m0, double inv_s0,
+                             float* __restrict__ mean32, float* __restrict__ corr16) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntrials) return;
+    double s = 0.0;
+    for (uint32_t i = 0; i < ntiles; i++) s += partial[(size_t)t * ntiles + i];
+    const double mean = s / (double)n;
+    mean32[t] = (float)mean;
+    corr16[t] = (float)((m0 - mean) * inv_s0);
+}
+
+void launch_trial_mean(hipStream_t s, const double* partial, uint32_t ntiles, uint32_t ntrials,
+                       uint32_t n, double m0, double inv_s0, float* mean32, float* corr16) {
+    hipLaunchKernelGGL(k_trial_mean, dim3((ntrials + 255) / 256), dim3(256), 0, s, partial,
+                       ntiles, ntrials, n, m0, inv_s0, mean32, corr16);
+}
+
+}  // namespace psr

@@ -1,0 +1,115 @@
+"""The C-ABI library on the CPU: it loads, exports every symbol of
+include/psr_gpu.h, and its host-side numerics (no GPU) agree with the oracle.
+No compute call here touches a GPU."""
+import json
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, ROOT
+from pulsarsr_amd import _ffi, dedisperse, fbank, search
+from pulsarsr_amd.core import PsrError
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "psr_gpu.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(psr_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _ffi.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert {n for n, _, _ in _ffi.EXPORTS} == set(names)
+
+
+def test_version_and_error_channel():
+    assert b"gfx950" in _ffi.lib().psr_version()
+    with pytest.raises(PsrError) as e:
+        dedisperse.delays([0.0], 1.0, 1.0)
+    assert e.value.code == _ffi.PSR_E_INVALID
+    assert "positive frequencies/tsamp and nonnegative DM required" in str(e.value)
+
+
+def test_delays_kat_and_errors():
+    assert dedisperse.delays([1000.0, 2000.0], 0.001, 1.0).tolist() == [3, 0]
+    with pytest.raises(PsrError):
+        dedisperse.plan(0.0, 5.0, 0.0)
+    with pytest.raises(PsrError, match="more than 100000 DM trials"):
+        dedisperse.plan(0.0, 1e6, 1.0)
+
+
+def test_delays_match_oracle_bit_exactly():
+    rng = np.random.default_rng(0)
+    for nch, lo, hi, ts in ((96, 1352.0, 1448.0, 72e-6), (1024, 1200.0, 1600.0, 64e-6), (33, 300.0, 400.0, 1e-4)):
+        freqs = np.sort(rng.uniform(lo, hi, nch))[::-1].copy()
+        for dm in rng.uniform(0, 1000, 20):
+            assert (dedisperse.delays(freqs, ts, dm) == oracle.delays(freqs, ts, dm)).all()
+
+
+def test_dm_step_plan_match_golden():
+    g = json.load(open(os.path.join(GOLDEN, "gbt_report.json")))["settings"]
+    h = json.load(open(os.path.join(GOLDEN, "headers.json")))["gbt_header"]
+    freqs = dedisperse.frequencies(h["fch1"], h["foff"], h["nchans"])
+    assert dedisperse.dm_step(freqs, h["tsamp"]) == g["dm_step"]
+    p = dedisperse.plan(g["dm_min"], g["dm_max"], g["dm_step"])
+    assert len(p) == g["trial_count"] and (p == oracle.plan(g["dm_min"], g["dm_max"], g["dm_step"])).all()
+    d = json.load(open(os.path.join(GOLDEN, "dm62_delays.json")))
+    assert dedisperse.delays(freqs, d["tsamp"], d["dm"]).tolist() == d["delays"]
+
+
+def _put(s):
+    return struct.pack("<i", len(s)) + s.encode()
+
+
+def _header(bits=8, extra=b""):
+    b = _put("HEADER_START")
+    for k, v in (("nchans", 2), ("nbits", bits), ("nifs", 1), ("data_type", 1)):
+        b += _put(k) + struct.pack("<i", v)
+    for k, v in (("tsamp", 0.001), ("fch1", 1400.0), ("foff", -1.0)):
+        b += _put(k) + struct.pack("<d", v)
+    return b + extra + _put("HEADER_END")
+
+
+def test_parse_header_fields():
+    b = _header(4) + bytes([0x21, 0x43])
+    h = fbank.parse_header(b)
+    assert (h.nchans, h.nbits, h.tsamp, h.fch1, h.foff) == (2, 4, 0.001, 1400.0, -1.0)
+    assert h.data_offset == len(b) - 2 and h.nsamples == 2
+    assert h.frequencies().tolist() == [1400.0, 1399.0]
+
+
+def test_parse_header_rejects_unknown_and_duplicate_keywords():
+    # crates/fbank/src/lib.rs:281-292
+    with pytest.raises(PsrError, match="unknown keyword"):
+        fbank.parse_header(_put("HEADER_START") + _put("surprise"))
+    with pytest.raises(PsrError, match="duplicate nbits"):
+        fbank.parse_header(_header(8, _put("nbits") + struct.pack("<i", 8)))
+    with pytest.raises(PsrError, match="missing HEADER_START"):
+        fbank.parse_header(_put("HEADER_STOP"))
+
+
+def test_sift_matches_oracle():
+    rng = np.random.default_rng(2)
+    cands = []
+    for _ in range(300):
+        f = float(rng.uniform(1, 400))
+        cands.append(search.Candidate(f, 1 / f, float(rng.choice([0.0, 10.0, 20.0])), int(rng.choice([1, 2, 4])),
+                                      float(np.round(rng.uniform(6, 50), 1))))
+    got = search.sift(cands, 0.05, 40, True)
+    ref = oracle.sift([c.__dict__ for c in cands], 0.05, 40, True)
+    assert [(c.frequency_hz, c.dm, c.score) for c in got] == [(c["frequency_hz"], c["dm"], c["score"]) for c in ref]
+
+
+def test_sift_reference_case():
+    # crates/search/src/lib.rs:204-226
+    mk = lambda f, dm, s: search.Candidate(f, 1 / f, dm, 1, s)  # noqa: E731
+    c = search.sift([mk(100.0, 20.0, 10.0), mk(200.0, 21.0, 8.0), mk(100.01, 19.0, 9.0), mk(137.0, 0.0, 20.0)],
+                    0.1, 10, True)
+    assert len(c) == 1 and c[0].dm == 20.0
