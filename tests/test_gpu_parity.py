@@ -1,0 +1,247 @@
+"""Parity of the gfx950 path (through the C ABI) against the oracle.
+
+Integer work (delay tables, dedispersed integer channel sums, fold bins and
+counts) must be bit-exact.  FP32 spectra are compared normwise (the FFT
+rounding of the reference, rustfft, is not reproducible; see the oracle header)
+with a 1e-5 relative bound, and candidate lists must agree on (harmonics,
+frequency) with scores within 1e-4 relative, allowing only threshold-edge flips.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, fold, search, synth
+from pulsarsr_amd.core import PsrError
+
+pytestmark = pytest.mark.gpu
+
+SCORE_RTOL = 1e-4
+
+
+def small_workload(nbits, nchans=48, nsamples=1 << 15, seed=3, pulsar_amp=None):
+    mean, sigma = {1: (0.5, 0.5), 2: (1.5, 0.8), 4: (8.0, 2.0), 8: (128.0, 16.0), 16: (30000.0, 3000.0),
+                   32: (100.0, 1.0)}[nbits]
+    amp = pulsar_amp if pulsar_amp is not None else synth.amplitude_for_snr(40.0, nchans, nsamples, 0.05)
+    return synth.Workload("test-%dbit" % nbits, seed, nsamples, nchans, nbits, 64e-6, 1500.0, -2.0, mean, sigma,
+                          pulsars=[(0.0123, 40.0, 0.05, amp, 0.3)], dms=np.arange(0, 80.0, 2.5),
+                          min_hz=10.0, threshold=7.0)
+
+
+def setup(w):
+    payload = synth.generate(w)
+    obs = dedisperse.Observation.from_packed(payload, w.nbits, w.nchans, w.tsamp, w.fch1, w.foff)
+    data, ns = oracle.decode(payload, w.nbits, w.nchans)
+    return payload, obs, data, ns
+
+
+def disc_len(w):
+    d = oracle.delays(w.frequencies(), w.tsamp, float(w.dms[-1]))
+    return (w.nsamples - 2 * int(d.max())) // 2
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8, 16, 32])
+def test_dedisperse_bit_exact(ctx, nbits):
+    w = small_workload(nbits)
+    _, obs, data, ns = setup(w)
+    n = disc_len(w)
+    got = dedisperse.apply_dms(obs, w.dms, n)
+    for i, dm in enumerate(w.dms):
+        ref = oracle.apply(data, ns, oracle.delays(w.frequencies(), w.tsamp, dm), n)
+        assert (got[i].view(np.uint32) == ref.view(np.uint32)).all(), "DM %g" % dm
+
+
+def test_dedisperse_full_length_and_unsorted_dms(ctx):
+    w = small_workload(8)
+    _, obs, data, ns = setup(w)
+    dms = np.array([30.0, 0.0, 77.5, 12.5])
+    got = dedisperse.apply_dms(obs, dms)  # None: full available length of the largest DM
+    d = oracle.delays(w.frequencies(), w.tsamp, 77.5)
+    assert got.shape[1] == ns - int(d.max())
+    for i, dm in enumerate(dms):
+        ref = oracle.apply(data, ns, oracle.delays(w.frequencies(), w.tsamp, dm), got.shape[1])
+        assert (got[i] == ref).all()
+
+
+def test_dispersed_delta_collapses(ctx):
+    # crates/dedisperse/src/lib.rs:95-108 through the GPU
+    for freqs in ([1400.0, 1300.0, 1200.0], [1200.0, 1300.0, 1400.0]):
+        d = dedisperse.delays(freqs, 0.001, 20.0)
+        n = 100
+        data = np.zeros(n * 3, np.float32)
+        for c, dl in enumerate(d):
+            data[c * n + 10 + int(dl)] = 1.0
+        obs = dedisperse.Observation.from_f32(data, n, 0.001, freqs[0], freqs[1] - freqs[0])
+        out = dedisperse.apply(obs, 20.0)
+        assert out[10] == 3.0 and out.sum() == 3.0
+
+
+def test_f32_channel_order_sums(ctx):
+    rng = np.random.default_rng(9)
+    nch, n = 24, 5000
+    data = (rng.standard_normal(nch * n) * 37.0 + 1000.0).astype(np.float32)
+    freqs = 1400.0 - np.arange(nch) * 3.0
+    obs = dedisperse.Observation.from_f32(data, n, 1e-3, 1400.0, -3.0)
+    dms = [0.0, 35.0, 70.0]
+    got = dedisperse.apply_dms(obs, dms, 4000)
+    for i, dm in enumerate(dms):
+        ref = oracle.apply(data, n, oracle.delays(freqs, 1e-3, dm), 4000)
+        assert (got[i].view(np.uint32) == ref.view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("n", [1000, 4096, 8000, 32583, 100000, 263500])
+def test_spectrum_matches_oracle(ctx, n):
+    rng = np.random.default_rng(n)
+    t = np.arange(n)
+    x = (rng.standard_normal(n) * 16 + 2000 + 30 * np.sin(2 * np.pi * t / 97.3)).round().astype(np.float32)
+    raw, white = search.spectrum(x, 64e-6, search.Config(min_hz=10.0))
+    oraw, owhite = oracle.spectrum(x, 64e-6, min_hz=10.0)
+    assert len(raw) == len(oraw)
+    assert np.linalg.norm(raw - oraw) <= 1e-5 * np.linalg.norm(oraw)
+    # the median-based scales are exact order statistics: compare bin by bin against
+    # the local whitening level (1e-5 of it)
+    assert np.max(np.abs(white - owhite)) <= 1e-4 * max(1.0, np.max(np.abs(owhite))) + 1e-5
+    assert np.linalg.norm(white - owhite) <= 1e-5 * np.linalg.norm(owhite)
+
+
+def match_lists(got, ref, threshold, max_candidates=32):
+    """Per-trial candidate lists agree up to threshold-edge flips; returns worst rel score error."""
+    worst = 0.0
+    for t, (g, r) in enumerate(zip(got, ref)):
+        rs = [(c["harmonics"], c["frequency_hz"], c["score"]) for c in r]
+        gs = [(c.harmonics, c.frequency_hz, c.score) for c in g]
+        used = set()
+        for h, f, s in gs:
+            hit = None
+            for j, (h2, f2, s2) in enumerate(rs):
+                if j not in used and h2 == h and abs(f2 - f) <= 1e-6 * max(1.0, f):
+                    hit = j
+                    break
+            if hit is None:
+                assert abs(s - threshold) < 1e-3 * threshold or len(gs) >= max_candidates, (t, h, f, s)
+                continue
+            used.add(hit)
+            worst = max(worst, abs(s - rs[hit][2]) / abs(rs[hit][2]))
+        for j, (h2, f2, s2) in enumerate(rs):
+            if j not in used:
+                assert abs(s2 - threshold) < 1e-3 * threshold or len(rs) >= max_candidates, (t, h2, f2, s2)
+    return worst
+
+
+@pytest.mark.parametrize("nbits", [4, 8])
+def test_search_matches_oracle(ctx, nbits):
+    w = small_workload(nbits, nsamples=1 << 16)
+    _, obs, data, ns = setup(w)
+    n = disc_len(w)
+    cfg = search.Config(min_hz=w.min_hz, max_hz=w.max_hz, threshold=w.threshold, max_harmonics=16)
+    got = search.search(obs, w.dms, n, cfg)
+    ref = oracle.search_trials(data, ns, w.frequencies(), w.tsamp, w.dms, n, threads=4, min_hz=w.min_hz,
+                               max_hz=w.max_hz, threshold=w.threshold)
+    assert sum(len(x) for x in ref) > 0
+    assert match_lists(got, ref, w.threshold) < SCORE_RTOL
+    # the injected pulsar is the strongest family
+    best = max((c for x in got for c in x), key=lambda c: c.score)
+    assert abs(best.frequency_hz * 0.0123 - round(best.frequency_hz * 0.0123)) < 0.05
+
+
+def test_periodicity_finds_sine_and_respects_birdies(ctx):
+    # crates/search/src/lib.rs:180-203
+    tsamp, f = 0.001, 31.25
+    x = np.sin(2 * np.pi * f * np.arange(8192) * tsamp).astype(np.float32)
+    c = search.periodicity(x, tsamp, 0.0, search.Config(min_hz=10.0))
+    assert abs(c[0].frequency_hz - f) < 0.1
+    ref = oracle.periodicity(x, tsamp, 0.0, min_hz=10.0)
+    assert match_lists([c], [ref], 6.0) < SCORE_RTOL
+    c = search.periodicity(x, tsamp, 0.0, search.Config(min_hz=10.0, birdies=[(f, 1.0)]))
+    assert all(abs(k.frequency_hz - f) > 1.0 for k in c)
+
+
+def test_off_bin_pulse_train_retains_fundamental(ctx):
+    # crates/search/src/lib.rs:228-247
+    from test_oracle_golden import xorshift_pulse_train
+    x, dt, period = xorshift_pulse_train()
+    c = search.periodicity(x, dt, 18.0)
+    kept = search.sift(c, 1.5 / (len(x) * dt), 10, False)
+    assert abs(kept[0].frequency_hz - 1.0 / period) < 0.2
+    ref = oracle.periodicity(x, dt, 18.0)
+    assert match_lists([c], [ref], 6.0) < SCORE_RTOL
+
+
+def test_search_is_deterministic(ctx):
+    w = small_workload(8, nsamples=1 << 16)
+    _, obs, _, _ = setup(w)
+    n = disc_len(w)
+    cfg = search.Config(min_hz=w.min_hz, threshold=w.threshold)
+    a, ca, _ = search.search_raw(obs, w.dms, n, cfg)
+    b, cb, _ = search.search_raw(obs, w.dms, n, cfg)
+    assert (ca == cb).all() and bytes(a) == bytes(b)
+
+
+def test_fp16_storage_error_vs_fp64_oracle(ctx):
+    w = small_workload(8, nsamples=1 << 16)
+    _, obs, data, ns = setup(w)
+    n = disc_len(w)
+    cfg = search.Config(min_hz=w.min_hz, threshold=w.threshold, precision=PRECISION_FP16)
+    got = search.search(obs, w.dms, n, cfg)
+    worst = 0.0
+    for i, dm in enumerate(w.dms):
+        x = oracle.apply(data, ns, oracle.delays(w.frequencies(), w.tsamp, dm), n)
+        ref = oracle.periodicity(x, w.tsamp, dm, f64=True, min_hz=w.min_hz, threshold=w.threshold)
+        for g in got[i][:3]:
+            hit = [r for r in ref if r["harmonics"] == g.harmonics and abs(r["frequency_hz"] - g.frequency_hz) < 1e-4 * g.frequency_hz]
+            if hit:
+                worst = max(worst, abs(g.score - hit[0]["score"]) / hit[0]["score"])
+    assert worst <= 1e-2
+
+
+def test_fold_bins_bit_exact(ctx):
+    rng = np.random.default_rng(4)
+    x = (rng.integers(0, 4000, 50000)).astype(np.float32)
+    periods = [0.0123, 0.0456, 0.1, 0.00462]
+    offs = [0, 17, 1000, 3]
+    got = fold.profiles_at(x, 64e-6, periods, 64, offs)
+    for g, p, o in zip(got, periods, offs):
+        r = oracle.profile_at(x, 64e-6, p, 64, o)
+        assert (g.counts == r["counts"]).all()
+        assert (g.bins == r["bins"]).all()
+        assert g.snr == r["snr"] and g.reduced_chi_square == r["reduced_chi_square"]
+
+
+def test_refine_matches_oracle(ctx):
+    i = np.arange(20000)
+    x = np.where(i % 100 < 5, 10.0, ((i * 17 % 23) - 11.0) / 10.0).astype(np.float32)
+    g = fold.refine(x, 0.001, 10.0, 0.05, 40, 32)
+    r = oracle.refine(x, 0.001, 10.0, 0.05, 40, 32)
+    assert g.period_s == r["period_s"] and g.reduced_chi_square == r["reduced_chi_square"]
+    a = fold.profile(x, 0.001, 0.1, 32)
+    b = fold.profile(x, 0.001, 0.1037, 32)
+    assert a.snr > b.snr * 2 and int(a.counts.sum()) == len(x)
+
+
+def test_error_paths(ctx):
+    w = small_workload(8)
+    _, obs, _, _ = setup(w)
+    with pytest.raises(PsrError, match="nonnegative DM"):
+        dedisperse.apply_dms(obs, [-1.0], 100)
+    with pytest.raises(PsrError, match="invalid output length"):
+        dedisperse.apply_dms(obs, [10.0], w.nsamples)
+    with pytest.raises(PsrError, match="delay exceeds observation"):
+        dedisperse.apply_dms(obs, [1e6], 100)
+    with pytest.raises(PsrError) as e:
+        search.periodicity(np.zeros(1, np.float32), 1e-3, 0.0)
+    assert e.value.code == -6
+
+
+def test_memory_cap_forces_oom(ctx):
+    from pulsarsr_amd import Context
+    c = Context(0)
+    w = small_workload(8)
+    payload = synth.generate(w)
+    obs = dedisperse.Observation.from_packed(payload, w.nbits, w.nchans, w.tsamp, w.fch1, w.foff, ctx=c)
+    c.set_memory_cap(1024)
+    with pytest.raises(PsrError) as e:
+        dedisperse.apply_dms(obs, w.dms, 1000)
+    assert e.value.code == -4
+    c.set_memory_cap(0)
+    assert dedisperse.apply_dms(obs, w.dms[:2], 1000).shape == (2, 1000)
