@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--cpu-trials", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32-steps", type=int, default=1)
+    ap.add_argument("--trials", type=int, default=0, help="use the first N DM trials only (profiling runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -88,6 +89,8 @@ def main():
 
     from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, synth
     w = synth.workload(a.workload)
+    if a.trials:
+        w.dms = w.dms[:a.trials]
     t0 = time.perf_counter()
     payload = synth.generate(w, threads=min(16, os.cpu_count() or 1))
     log("generated %.2f GB in %.1f s" % (payload.nbytes / 1e9, time.perf_counter() - t0))

@@ -39,8 +39,8 @@ struct psr_ctx {
     hipStream_t stream = nullptr;
     uint64_t mem_cap = 0;
     uint64_t mem_used = 0;
-    DevBuf delays, dms, series, dd_partial, mean32, corr16, work, spec_t, raw_nat, scales,
-        fft_partial, peaks, peak_counts, cands, counts, twiddles, birdies, fold_series, fold_per,
+    DevBuf delays, dd_table, dms, series, dd_partial, mean32, corr16, work, spec_t, raw_nat, scales,
+        fft_partial, peaks, peak_counts, cands, counts, twiddles, costabs, birdies, fold_series, fold_per,
         fold_off, fold_sums, fold_counts;
     SpecGeom tw_geom{};
     bool tw_valid = false;
@@ -137,6 +137,13 @@ uint32_t spread_bound(const psr_obs* o, const double* dms_sorted, size_t ntrials
 }
 
 int upload_twiddles(psr_ctx* c, const SpecGeom& g) {
+    if (!c->costabs.p) {
+        std::vector<float> ct(COSTAB_SIZE);
+        fill_costabs(ct.data());
+        int rc = ensure(c, c->costabs, ct.size() * sizeof(float));
+        if (rc) return rc;
+        HIP_TRY(hipMemcpy(c->costabs.p, ct.data(), ct.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     if (c->tw_valid && c->tw_geom.m == g.m && c->tw_geom.m1 == g.m1 && c->tw_geom.nfft == g.nfft)
         return PSR_OK;
     const size_t nt = twiddle_table_size(g);
@@ -246,7 +253,7 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
     if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
     launch_fft(s, g, c->series.p, (int)p.fp16, (const float*)c->mean32.p, (const float*)c->corr16.p,
                nt, (float2*)c->work.p, c->spec_t.p, (double*)c->fft_partial.p,
-               (const float2*)c->twiddles.p);
+               (const float2*)c->twiddles.p, (const float*)c->costabs.p);
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
     launch_whiten(s, g, c->spec_t.p, (int)p.fp16, nt, (const double*)c->fft_partial.p, npart,
@@ -316,9 +323,9 @@ void psr_ctx_destroy(psr_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf* all[] = {&c->delays, &c->dms, &c->series, &c->dd_partial, &c->mean32, &c->corr16,
+    DevBuf* all[] = {&c->delays, &c->dd_table, &c->dms, &c->series, &c->dd_partial, &c->mean32, &c->corr16,
                      &c->work, &c->spec_t, &c->raw_nat, &c->scales, &c->fft_partial, &c->peaks,
-                     &c->peak_counts, &c->cands, &c->counts, &c->twiddles, &c->birdies,
+                     &c->peak_counts, &c->cands, &c->counts, &c->twiddles, &c->costabs, &c->birdies,
                      &c->fold_series, &c->fold_per, &c->fold_off, &c->fold_sums, &c->fold_counts};
     for (DevBuf* b : all) release(c, *b);
     for (int i = 0; i < 8; i++)
@@ -573,9 +580,11 @@ static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const doub
     const double inv_s0 = 1.0 / o->s0;
     if (int_path) {
         const uint32_t spread = spread_bound(o, dms_host, nt, 32);
+        if ((rc = ensure(c, c->dd_table, dedisp_u8_table_words(o->nchans, nt) * sizeof(int32_t)))) return rc;
         launch_dedisp_u8(s, (const uint8_t*)o->data, o->pitch, o->nchans, o->maxval,
-                         (const int32_t*)c->delays.p, nt, n, c->series.p, g.series_pitch, fp16,
-                         (float)o->m0, (float)inv_s0, (double*)c->dd_partial.p, spread);
+                         (const int32_t*)c->delays.p, (int32_t*)c->dd_table.p, nt, n, c->series.p,
+                         g.series_pitch, fp16, (float)o->m0, (float)inv_s0, (double*)c->dd_partial.p,
+                         spread);
     } else {
         launch_dedisp_f32(s, o->data, o->kind, o->pitch, o->nchans, (const int32_t*)c->delays.p, nt,
                           n, c->series.p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,

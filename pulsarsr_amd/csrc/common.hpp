@@ -67,9 +67,10 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
                    double tsamp, const double* dms, uint32_t ntrials, int32_t* out);
 // integer dedispersion (u8 rows), outputs f32 exact sums or FP16 scaled
 void launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
-                      uint32_t maxval, const int32_t* delays, uint32_t ntrials, uint32_t n,
-                      void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                      uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
+                      uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
                       double* partial, uint32_t spread_max);
+size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials);
 // sequential-order f32 dedispersion (16-bit and float rows)
 void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch,
                        uint32_t nchans, const int32_t* delays, uint32_t ntrials, uint32_t n,
@@ -78,7 +79,16 @@ void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch
 // FFT + power
 void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
                 const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
-                void* spec_t, double* partial_sums, const float2* tw);
+                void* spec_t, double* partial_sums, const float2* tw, const float* cos_tables);
+
+// Quarter-wave cosine tables C_L[j] = cos(2 pi j / L), j = 0..L/4, for every
+// power of two L = 2..COSTAB_MAX, concatenated in increasing L.
+constexpr int COSTAB_MAX = 8192;
+__host__ __device__ constexpr int costab_offset(int L) {
+    return L <= 2 ? 0 : costab_offset(L / 2) + (L / 2) / 4 + 1;
+}
+constexpr int COSTAB_SIZE = costab_offset(2 * COSTAB_MAX);
+void fill_costabs(float* host_table);
 // whitening scales + natural-order raw spectrum
 void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp16,
                    uint32_t ntrials, const double* partial_sums, uint32_t npartials,

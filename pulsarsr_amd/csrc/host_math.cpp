@@ -254,16 +254,11 @@ SpecGeom make_geometry(uint32_t n) {
     return g;
 }
 
-// Twiddle tables in FP64, stored FP32 (layout documented in fft kernels):
-//  [0, L2)              W_{m2}^j           first-pass (or single-pass) butterflies
-//  [L2, L2+L1)          W_{m1}^j           second-pass butterflies
-//  [.., +2048)          W_m^j, j<2048      four-step twiddle, low part
-//  [.., +m/2048)        W_m^(2048 j)       four-step twiddle, high part
-//  [.., +m2)            W_nfft^j           r2c post-processing, low part
-//  [.., +m1)            W_nfft^(m2 j)      r2c post-processing, high part
+// Geometry-specific twiddles in FP64, stored FP32 (used only by the four-step
+// path):  [W_m^j, j < 2048][W_m^(2048 j), j < m/2048][W_nfft^k2, k2 < m2]
 size_t twiddle_table_size(const SpecGeom& g) {
     const size_t hi = g.m >= 2048 ? g.m / 2048 : 1;
-    return (size_t)g.m2 + g.m1 + 2048 + hi + g.m2 + g.m1;
+    return 2048 + hi + (size_t)g.m2;
 }
 static void put(float2* t, size_t i, double turns) {
     const double a = -2.0 * M_PI * turns;
@@ -272,18 +267,19 @@ static void put(float2* t, size_t i, double turns) {
 }
 void fill_twiddles(const SpecGeom& g, float2* t) {
     size_t o = 0;
-    for (uint32_t j = 0; j < g.m2; j++) put(t, o + j, (double)j / g.m2);
-    o += g.m2;
-    for (uint32_t j = 0; j < g.m1; j++) put(t, o + j, (double)j / g.m1);
-    o += g.m1;
     for (uint32_t j = 0; j < 2048; j++) put(t, o + j, (double)j / g.m);
     o += 2048;
     const size_t hi = g.m >= 2048 ? g.m / 2048 : 1;
     for (size_t j = 0; j < hi; j++) put(t, o + j, (double)(2048.0 * j) / g.m);
     o += hi;
     for (uint32_t j = 0; j < g.m2; j++) put(t, o + j, (double)j / g.nfft);
-    o += g.m2;
-    for (uint32_t j = 0; j < g.m1; j++) put(t, o + j, (double)((double)g.m2 * j) / g.nfft);
+}
+void fill_costabs(float* t) {
+    for (int L = 2; L <= COSTAB_MAX; L *= 2) {
+        const int off = costab_offset(L);
+        for (int j = 0; j <= L / 4; j++) t[off + j] = (float)std::cos(2.0 * M_PI * (double)j / (double)L);
+        t[off + L / 4] = 0.0f;  // cos(pi/2)
+    }
 }
 
 }  // namespace psr

@@ -138,16 +138,19 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 }
 
 // ------------------------------------------------- integer dedispersion (u8)
-// Workgroup = 4 waves sharing one time tile of DD_TILE outputs and a block of
-// DD_TRIALS consecutive DM trials (each wave owns DD_M trials).  Channels are
-// staged DD chunk by chunk into LDS as 32-bit words
-//     w[i] = s[base + i] | s[base + i + 64] << 16,
-// so one aligned ds_read_b32 at any delay offset yields the samples of two
-// outputs (t and t+64) of one lane, accumulated with a single v_pk_add_u16.
-// Lane l of a wave owns outputs t0 + l + 64*q, q = 0..2*DD_G-1.
+// Workgroup = 4 waves sharing one time tile of DD_TILE = 512 outputs and a block
+// of DD_TRIALS consecutive DM trials (each wave owns DD_M trials).  Channels are
+// staged chunk by chunk into LDS as 32-bit words
+//     w[i] = s[base + i] | s[base + i + 128] << 16,
+// so any delay offset is an aligned dword address.  Lane l reads the adjacent
+// words e + 256g + 2l and e + 256g + 2l + 1 (g = 0, 1) with one ds_read2_b32 per
+// g -- the 64 lanes of a wave touch 128 distinct consecutive dwords, free of
+// bank conflicts -- and accumulates each word (two outputs, 128 apart) with a
+// single v_pk_add_u16.  Lane l owns outputs 256g + 2l + {0, 1, 128, 129}.
+// The per-trial LDS byte offsets come from a table built by k_dd_offsets and
+// are read with scalar loads.
 constexpr int DD_M = 8;
-constexpr int DD_G = 4;
-constexpr int DD_TILE = 128 * DD_G;
+constexpr int DD_TILE = 512;
 constexpr int DD_TRIALS = 4 * DD_M;
 
 __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
@@ -155,75 +158,95 @@ __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, x + y);
 }
 
+// bases[blk][c] = delay of the block's first trial rounded down to 4 samples;
+// offs[blk][wave][c][m] = 4 * (delay of trial wave*DD_M + m - bases[blk][c]),
+// the LDS byte offset; a wave's offsets of consecutive channels are contiguous.
+__global__ __launch_bounds__(256) void k_dd_offsets(const int32_t* __restrict__ delays,
+                                                    uint32_t nchans, uint32_t ntrials,
+                                                    int32_t* __restrict__ bases,
+                                                    int32_t* __restrict__ offs) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
+    if (i >= (uint64_t)nblk * nchans) return;
+    const uint32_t blk = (uint32_t)(i / nchans), c = (uint32_t)(i % nchans);
+    const uint32_t d0 = blk * DD_TRIALS;
+    const int32_t base = delays[(size_t)d0 * nchans + c] & ~3;
+    bases[i] = base;
+    for (int m = 0; m < DD_TRIALS; m++) {
+        const uint32_t d = min(d0 + m, ntrials - 1);
+        const uint32_t wv = m / DD_M, mm = m % DD_M;
+        offs[(((size_t)blk * 4 + wv) * nchans + c) * DD_M + mm] = 4 * (delays[(size_t)d * nchans + c] - base);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_dedisp_u8(
     const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_chans,
-    const int32_t* __restrict__ delays, uint32_t ntrials, uint32_t n, void* __restrict__ out,
-    uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial,
-    uint32_t W, uint32_t C) {
-    extern __shared__ uint32_t lds[];
-    uint32_t* rows = lds;                            // [C][W]
-    int32_t* dl = (int32_t*)(lds + (size_t)C * W);   // [C][DD_TRIALS]
+    const int32_t* __restrict__ bases, const int32_t* __restrict__ offs, uint32_t ntrials,
+    uint32_t n, void* __restrict__ out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+    double* __restrict__ partial, uint32_t W, uint32_t C) {
+    extern __shared__ uint32_t lds[];  // [C][W] words
     const uint32_t tile = blockIdx.x;
     const uint32_t t0 = tile * DD_TILE;
-    const uint32_t d0 = blockIdx.y * DD_TRIALS;
+    const uint32_t blk = blockIdx.y;
+    const uint32_t d0 = blk * DD_TRIALS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    const int32_t* bblk = bases + (size_t)blk * nchans;
+    const int32_t* oblk = offs + ((size_t)blk * 4 + wave) * nchans * DD_M;
 
-    uint32_t acc16[DD_M][DD_G];
-    uint32_t acc32[DD_M][2 * DD_G];
+    // acc16[m][q]: word q of trial m (q = 2g + j: outputs 256g + 2l + j and +128)
+    uint32_t acc16[DD_M][4];
+    uint32_t acc32[DD_M][8];
 #pragma unroll
     for (int m = 0; m < DD_M; m++) {
 #pragma unroll
-        for (int p = 0; p < DD_G; p++) acc16[m][p] = 0;
+        for (int q = 0; q < 4; q++) acc16[m][q] = 0;
 #pragma unroll
-        for (int q = 0; q < 2 * DD_G; q++) acc32[m][q] = 0;
+        for (int q = 0; q < 8; q++) acc32[m][q] = 0;
     }
     uint32_t in_phase = 0;
     const uint32_t quads = W / 4;
     for (uint32_t cc = 0; cc < nchans; cc += C) {
         const uint32_t nc = min(C, nchans - cc);
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nc * DD_TRIALS; i += blockDim.x) {
-            const uint32_t c = i / DD_TRIALS, tt = i % DD_TRIALS;
-            const uint32_t d = min(d0 + tt, ntrials - 1);
-            dl[c * DD_TRIALS + tt] = delays[(size_t)d * nchans + cc + c];
-        }
-        __syncthreads();
         for (uint32_t i = threadIdx.x; i < nc * quads; i += blockDim.x) {
-            const uint32_t c = i / quads, q = i % quads;
-            const uint32_t base = (uint32_t)dl[c * DD_TRIALS] & ~3u;
+            const uint32_t c = i / quads, q = i - c * quads;
+            const uint32_t base = (uint32_t)bblk[cc + c];
             const uint8_t* row = data + (size_t)(cc + c) * pitch;
             const uint64_t off = (uint64_t)t0 + base + 4u * q;
             const uint32_t a = off + 4 <= pitch ? *(const uint32_t*)(row + off) : 0u;
-            const uint32_t b = off + 68 <= pitch ? *(const uint32_t*)(row + off + 64) : 0u;
+            const uint32_t b = off + 132 <= pitch ? *(const uint32_t*)(row + off + 128) : 0u;
             uint4 w;
             w.x = __builtin_amdgcn_perm(b, a, 0x0c040c00u);
             w.y = __builtin_amdgcn_perm(b, a, 0x0c050c01u);
             w.z = __builtin_amdgcn_perm(b, a, 0x0c060c02u);
             w.w = __builtin_amdgcn_perm(b, a, 0x0c070c03u);
-            *(uint4*)(rows + (size_t)c * W + 4 * q) = w;
+            *(uint4*)(lds + (size_t)c * W + 4 * q) = w;
         }
         __syncthreads();
+        const char* rows = (const char*)lds + 8 * lane;
+#pragma unroll 2
         for (uint32_t c = 0; c < nc; c++) {
-            const int32_t* dc = dl + c * DD_TRIALS;
-            const int32_t base = dc[0] & ~3;
-            const uint32_t* rowc = rows + (size_t)c * W + lane;
+            const int32_t* oc = oblk + (size_t)(cc + c) * DD_M;
+            const char* rowc = rows + (size_t)c * W * 4;
 #pragma unroll
             for (int m = 0; m < DD_M; m++) {
-                const int e = __builtin_amdgcn_readfirstlane(dc[wave * DD_M + m] - base);
-                const uint32_t* r = rowc + e;
-#pragma unroll
-                for (int p = 0; p < DD_G; p++) acc16[m][p] = pk_add(acc16[m][p], r[128 * p]);
+                const uint32_t* r0 = (const uint32_t*)(rowc + oc[m]);
+                const uint32_t* r1 = (const uint32_t*)(rowc + oc[m] + 1024);
+                acc16[m][0] = pk_add(acc16[m][0], r0[0]);
+                acc16[m][1] = pk_add(acc16[m][1], r0[1]);
+                acc16[m][2] = pk_add(acc16[m][2], r1[0]);
+                acc16[m][3] = pk_add(acc16[m][3], r1[1]);
             }
             if (++in_phase == phase_chans) {
                 in_phase = 0;
 #pragma unroll
                 for (int m = 0; m < DD_M; m++)
 #pragma unroll
-                    for (int p = 0; p < DD_G; p++) {
-                        acc32[m][2 * p] += acc16[m][p] & 0xffffu;
-                        acc32[m][2 * p + 1] += acc16[m][p] >> 16;
-                        acc16[m][p] = 0;
+                    for (int q = 0; q < 4; q++) {
+                        acc32[m][2 * q] += acc16[m][q] & 0xffffu;
+                        acc32[m][2 * q + 1] += acc16[m][q] >> 16;
+                        acc16[m][q] = 0;
                     }
             }
         }
@@ -231,9 +254,9 @@ __global__ __launch_bounds__(256) void k_dedisp_u8(
 #pragma unroll
     for (int m = 0; m < DD_M; m++)
 #pragma unroll
-        for (int p = 0; p < DD_G; p++) {
-            acc32[m][2 * p] += acc16[m][p] & 0xffffu;
-            acc32[m][2 * p + 1] += acc16[m][p] >> 16;
+        for (int q = 0; q < 4; q++) {
+            acc32[m][2 * q] += acc16[m][q] & 0xffffu;
+            acc32[m][2 * q + 1] += acc16[m][q] >> 16;
         }
     const uint32_t ntiles = gridDim.x;
     const int m0i = (int)m0;
@@ -243,16 +266,20 @@ __global__ __launch_bounds__(256) void k_dedisp_u8(
         double lsum = 0.0;
         if (d < ntrials) {
 #pragma unroll
-            for (int q = 0; q < 2 * DD_G; q++) {
-                const uint32_t t = t0 + lane + 64 * q;
-                const uint32_t v = acc32[m][(q >> 1) * 2 + (q & 1)];
-                if (t < n) {
-                    lsum += (double)v;
-                    if (fp16)
-                        ((__half*)out)[(size_t)d * out_pitch + t] =
-                            __float2half_rn((float)((int)v - m0i) * inv_s0);
-                    else
-                        ((float*)out)[(size_t)d * out_pitch + t] = (float)v;
+            for (int q = 0; q < 4; q++) {
+                const int g = q >> 1, j = q & 1;
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++) {
+                    const uint32_t t = t0 + 256 * g + 2 * lane + j + 128 * hh;
+                    const uint32_t v = acc32[m][2 * q + hh];
+                    if (t < n) {
+                        lsum += (double)v;
+                        if (fp16)
+                            ((__half*)out)[(size_t)d * out_pitch + t] =
+                                __float2half_rn((float)((int)v - m0i) * inv_s0);
+                        else
+                            ((float*)out)[(size_t)d * out_pitch + t] = (float)v;
+                    }
                 }
             }
         }
@@ -263,25 +290,32 @@ __global__ __launch_bounds__(256) void k_dedisp_u8(
     }
 }
 
+size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
+    const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
+    return nblk * nchans * (1 + DD_TRIALS);
+}
+
 void launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
-                      uint32_t maxval, const int32_t* delays, uint32_t ntrials, uint32_t n,
-                      void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                      uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
+                      uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
                       double* partial, uint32_t spread_max) {
+    const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
+    int32_t* bases = table;
+    int32_t* offs = table + nblk * nchans;
+    hipLaunchKernelGGL(k_dd_offsets, dim3((unsigned)((nblk * nchans + 255) / 256)), dim3(256), 0, s,
+                       delays, nchans, ntrials, bases, offs);
     // rows must cover the largest in-block delay spread, 3 bytes of alignment
-    // slack, and the lane/pair footprint of the tile.
-    uint32_t W = spread_max + 3 + 64 + 128 * (DD_G - 1) + 1;
+    // slack, and the footprint of a trial's reads (words 0..383 past its offset).
+    uint32_t W = spread_max + 3 + 384 + 1;
     W = (W + 3) & ~3u;
-    const uint32_t budget = 48 * 1024;
+    const uint32_t budget = 40 * 1024;
     uint32_t C = 64;
-    while (C > 1 && (size_t)C * (W * 4 + DD_TRIALS * 4) > budget) C >>= 1;
+    while (C > 1 && (size_t)C * W * 4 > budget) C >>= 1;
     uint32_t phase = 65535u / (maxval ? maxval : 1);
     phase = phase > 4096 ? 4096 : phase;
-    // phases are whole numbers of chunks
-    phase = (phase / C) * C;
-    if (phase == 0) phase = 1;
-    const size_t shm = (size_t)C * (W * 4 + DD_TRIALS * 4);
-    dim3 grid((n + DD_TILE - 1) / DD_TILE, (ntrials + DD_TRIALS - 1) / DD_TRIALS);
-    hipLaunchKernelGGL(k_dedisp_u8, grid, dim3(256), shm, s, data, pitch, nchans, phase, delays,
+    const size_t shm = (size_t)C * W * 4;
+    dim3 grid((n + DD_TILE - 1) / DD_TILE, (unsigned)nblk);
+    hipLaunchKernelGGL(k_dedisp_u8, grid, dim3(256), shm, s, data, pitch, nchans, phase, bases, offs,
                        ntrials, n, out, out_pitch, fp16, m0, inv_s0, partial, W, C);
 }
 

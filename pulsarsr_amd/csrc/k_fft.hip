@@ -11,9 +11,13 @@
 //           split (the partner of k = k2 + m2*k1 is M-k, held in the same
 //           workgroup) and the power, written as spec_t[k2*m1 + k1].
 // The transposed spectrum is put back in natural order by the whitening kernel.
-// Butterflies are radix-4 Stockham (one radix-2 stage for odd log2 lengths) in
-// FP32 with FP64-generated twiddles; the upper half of the padded input is
-// zero and is never read from memory.
+//
+// The in-LDS transforms are Stockham passes of radix 8 or 16 (2 or 4 for the
+// remainder): each thread loads R points, applies the pass twiddles, runs a
+// register DFT_R and stores the outputs in autosorted order, so a 4096-point
+// transform takes three LDS round trips.  Butterflies are FP32; every twiddle
+// comes from FP64-generated tables (LDS-resident per workgroup).  The
+// upper half of the zero-padded input is never read from memory.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -26,94 +30,238 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
-// In-LDS Stockham FFT of V contiguous vectors of length L = 2^logL.
-// Each thread owns up to BPT radix-4 butterflies per stage; all of a stage's
-// inputs are read into registers before any output is written.
-template <int BPT>
-__device__ void lds_fft(float2* buf, int L, int logL, int V, const float2* __restrict__ tw) {
-    const int nt = blockDim.x;
-    const int tid = threadIdx.x;
-    int Ns = 1;
-    if (logL & 1) {
-        const int h = L >> 1;
-        float2 a0[2 * BPT], a1[2 * BPT];
+// ------------------------------------------------------------ DFT codelets
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+    const float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    const float2 s0 = cadd(a0, a2), d0 = csub(a0, a2);
+    const float2 s1 = cadd(a1, a3), d1 = csub(a1, a3);
+    a0 = cadd(s0, s1);
+    a2 = csub(s0, s1);
+    a1 = make_float2(d0.x + d1.y, d0.y - d1.x);  // d0 - i d1
+    a3 = make_float2(d0.x - d1.y, d0.y + d1.x);  // d0 + i d1
+}
+
+constexpr float RS2 = 0.70710678118654752440f;  // sqrt(1/2)
+constexpr float C16 = 0.92387953251128675613f;  // cos(pi/8)
+constexpr float S16 = 0.38268343236508977173f;  // sin(pi/8)
+
+template <int R>
+__device__ __forceinline__ void dft(float2* x);
+
+template <>
+__device__ __forceinline__ void dft<2>(float2* x) {
+    dft2(x[0], x[1]);
+}
+template <>
+__device__ __forceinline__ void dft<4>(float2* x) {
+    dft4(x[0], x[1], x[2], x[3]);
+}
+// n = a + 2m, k = b + 4c:  X[b+4c] = sum_a W2^(ac) W8^(ab) DFT4_m(x[a+2m])[b]
+template <>
+__device__ __forceinline__ void dft<8>(float2* x) {
+    float2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    float2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    o1 = make_float2(RS2 * (o1.x + o1.y), RS2 * (o1.y - o1.x));   // * W8^1
+    o2 = mul_mi(o2);                                               // * W8^2
+    o3 = make_float2(RS2 * (o3.y - o3.x), -RS2 * (o3.x + o3.y));  // * W8^3
+    x[0] = cadd(e0, o0);
+    x[4] = csub(e0, o0);
+    x[1] = cadd(e1, o1);
+    x[5] = csub(e1, o1);
+    x[2] = cadd(e2, o2);
+    x[6] = csub(e2, o2);
+    x[3] = cadd(e3, o3);
+    x[7] = csub(e3, o3);
+}
+// n = a + 4m, k = b + 4c:  X[b+4c] = DFT4_a(W16^(ab) DFT4_m(x[a+4m])[b])[c]
+template <>
+__device__ __forceinline__ void dft<16>(float2* x) {
+    float2 y[4][4];
 #pragma unroll
-        for (int i = 0; i < 2 * BPT; i++) {
-            const int j = tid + i * nt;
-            if (j < V * h) {
-                const int v = j / h, jj = j - v * h;
-                a0[i] = buf[v * L + jj];
-                a1[i] = buf[v * L + jj + h];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 2 * BPT; i++) {
-            const int j = tid + i * nt;
-            if (j < V * h) {
-                const int v = j / h, jj = j - v * h;
-                const int base = v * L + 2 * jj;
-                buf[base] = cadd(a0[i], a1[i]);
-                buf[base + 1] = csub(a0[i], a1[i]);
-            }
-        }
-        __syncthreads();
-        Ns = 2;
+    for (int a = 0; a < 4; a++) {
+        y[a][0] = x[a];
+        y[a][1] = x[a + 4];
+        y[a][2] = x[a + 8];
+        y[a][3] = x[a + 12];
+        dft4(y[a][0], y[a][1], y[a][2], y[a][3]);
     }
-    const int q = L >> 2;
-    for (; Ns < L; Ns <<= 2) {
-        float2 a[BPT][4];
+    // W16^j = (cos 2pi j/16, -sin 2pi j/16)
+    y[1][1] = cmul(y[1][1], make_float2(C16, -S16));
+    y[1][2] = cmul(y[1][2], make_float2(RS2, -RS2));
+    y[1][3] = cmul(y[1][3], make_float2(S16, -C16));
+    y[2][1] = cmul(y[2][1], make_float2(RS2, -RS2));
+    y[2][2] = mul_mi(y[2][2]);
+    y[2][3] = cmul(y[2][3], make_float2(-RS2, -RS2));
+    y[3][1] = cmul(y[3][1], make_float2(S16, -C16));
+    y[3][2] = cmul(y[3][2], make_float2(-RS2, -RS2));
+    y[3][3] = cmul(y[3][3], make_float2(-C16, S16));
 #pragma unroll
-        for (int i = 0; i < BPT; i++) {
-            const int j = tid + i * nt;
-            if (j < V * q) {
-                const int v = j / q, jj = j - v * q;
-#pragma unroll
-                for (int r = 0; r < 4; r++) a[i][r] = buf[v * L + jj + r * q];
-            }
-        }
-        __syncthreads();
-        const int tstep = L / (4 * Ns);
-#pragma unroll
-        for (int i = 0; i < BPT; i++) {
-            const int j = tid + i * nt;
-            if (j < V * q) {
-                const int v = j / q, jj = j - v * q;
-                const int k = jj & (Ns - 1);
-                float2 x0 = a[i][0], x1 = a[i][1], x2 = a[i][2], x3 = a[i][3];
-                if (Ns > 1) {
-                    x1 = cmul(x1, tw[k * tstep]);
-                    x2 = cmul(x2, tw[2 * k * tstep]);
-                    x3 = cmul(x3, tw[3 * k * tstep]);
-                }
-                const float2 s0 = cadd(x0, x2), d0 = csub(x0, x2);
-                const float2 s1 = cadd(x1, x3), d1 = csub(x1, x3);
-                const int base = v * L + (jj / Ns) * 4 * Ns + k;
-                buf[base] = cadd(s0, s1);
-                buf[base + 2 * Ns] = csub(s0, s1);
-                // forward: X1 = d0 - i d1, X3 = d0 + i d1
-                buf[base + Ns] = make_float2(d0.x + d1.y, d0.y - d1.x);
-                buf[base + 3 * Ns] = make_float2(d0.x - d1.y, d0.y + d1.x);
-            }
-        }
-        __syncthreads();
+    for (int b = 0; b < 4; b++) {
+        float2 a0 = y[0][b], a1 = y[1][b], a2 = y[2][b], a3 = y[3][b];
+        dft4(a0, a1, a2, a3);
+        x[b] = a0;
+        x[b + 4] = a1;
+        x[b + 8] = a2;
+        x[b + 12] = a3;
     }
 }
 
-struct TwOffsets {
-    uint32_t p1, p2, lo, hi, rlo, rhi;
-};
-static TwOffsets tw_offsets(const SpecGeom& g) {
-    TwOffsets o;
-    o.p1 = 0;
-    o.p2 = g.m2;
-    o.lo = g.m2 + g.m1;
-    o.hi = o.lo + 2048;
-    const uint32_t hi = g.m >= 2048 ? g.m / 2048 : 1;
-    o.rlo = o.hi + hi;
-    o.rhi = o.rlo + g.m2;
-    return o;
+// ------------------------------------------------------------- twiddles
+// W_L^m from a quarter-wave table C[j] = cos(2 pi j / L), j = 0..L/4.
+template <int L>
+__device__ __forceinline__ float2 twiddle(const float* C, int m) {
+    constexpr int Q = L / 4;
+    const int q = (m / Q) & 3, r = m & (Q - 1);
+    const float c0 = C[r], s0 = C[Q - r];  // cos, sin of 2 pi r / L
+    float c, s;
+    if (q == 0) {
+        c = c0;
+        s = s0;
+    } else if (q == 1) {
+        c = -s0;
+        s = c0;
+    } else if (q == 2) {
+        c = -c0;
+        s = -s0;
+    } else {
+        c = s0;
+        s = -c0;
+    }
+    return make_float2(c, -s);
+}
+
+// ------------------------------------------------------------ LDS FFTs
+// Element e of vector v lives at LDS index e*V + v (padded by one slot per 16
+// when PAD, which breaks the power-of-two strides of the V <= 2 layouts).
+template <int V, bool PAD>
+__device__ __forceinline__ int lidx(int v, int e) {
+    const int a = e * V + v;
+    return PAD ? a + (a >> 4) : a;
+}
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+constexpr int npasses(int logl) { return logl <= 4 ? 1 : (logl + 3) / 4; }
+constexpr int pass_bits(int logl, int p) {
+    return logl / npasses(logl) + (p < logl % npasses(logl) ? 1 : 0);
+}
+
+// Pass twiddles: pass p (span NS_p, radix R_p) multiplies input r of butterfly
+// k by W_(NS_p R_p)^(r k).  Each pass has two small LDS tables indexed by k,
+// W^k and W^(4k), so neighbouring lanes read neighbouring entries; the other
+// powers are products of these (at most three complex products deep).
+constexpr int ns_before(int logl, int p) {
+    return p == 0 ? 1 : ns_before(logl, p - 1) << pass_bits(logl, p - 1);
+}
+constexpr int tw_offset(int logl, int p) {  // entries of passes 1..p-1 (pass 0 has none)
+    return p <= 1 ? 0 : tw_offset(logl, p - 1) + ns_before(logl, p - 1);
+}
+constexpr int tw_total(int L) { return tw_offset(ilog2c(L), npasses(ilog2c(L))); }
+
+template <int L>
+__device__ __forceinline__ void build_pass_tw(float2* T1, float2* T4, const float* cos_tables) {
+    constexpr int LOGL = ilog2c(L);
+    const float* C = cos_tables + costab_offset(L);
+    constexpr int total = tw_total(L);
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        int p = 1;
+        while (p + 1 < npasses(LOGL) && i >= tw_offset(LOGL, p + 1)) p++;
+        const int k = i - tw_offset(LOGL, p);
+        const int step = L / (ns_before(LOGL, p) << pass_bits(LOGL, p));
+        T1[i] = twiddle<L>(C, (k * step) & (L - 1));
+        T4[i] = twiddle<L>(C, (4 * k * step) & (L - 1));
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void apply_tw(float2* x, float2 w1, float2 w4) {
+    if constexpr (R == 2) {
+        x[1] = cmul(x[1], w1);
+    } else {
+        const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        x[1] = cmul(x[1], w1);
+        x[2] = cmul(x[2], w2);
+        x[3] = cmul(x[3], w3);
+        if constexpr (R >= 8) {
+            x[4] = cmul(x[4], w4);
+            x[5] = cmul(x[5], cmul(w4, w1));
+            x[6] = cmul(x[6], cmul(w4, w2));
+            x[7] = cmul(x[7], cmul(w4, w3));
+        }
+        if constexpr (R == 16) {
+            const float2 w8 = cmul(w4, w4);
+            x[8] = cmul(x[8], w8);
+            x[9] = cmul(x[9], cmul(w8, w1));
+            x[10] = cmul(x[10], cmul(w8, w2));
+            x[11] = cmul(x[11], cmul(w8, w3));
+            const float2 w12 = cmul(w8, w4);
+            x[12] = cmul(x[12], w12);
+            x[13] = cmul(x[13], cmul(w12, w1));
+            x[14] = cmul(x[14], cmul(w12, w2));
+            x[15] = cmul(x[15], cmul(w12, w3));
+        }
+    }
+}
+
+constexpr int FFT_THREADS = 512;
+
+template <int L, int V, bool PAD, int R, int NS>
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* T1, const float2* T4) {
+    constexpr int NT = FFT_THREADS;
+    constexpr int NBF = V * L / R;
+    constexpr int PER = (NBF + NT - 1) / NT;
+    float2 x[PER][R];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int j = tid + i * NT;
+        if (NBF % NT == 0 || j < NBF) {
+            const int v = j % V, jj = j / V;
+#pragma unroll
+            for (int r = 0; r < R; r++) x[i][r] = buf[lidx<V, PAD>(v, jj + r * (L / R))];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int j = tid + i * NT;
+        if (NBF % NT == 0 || j < NBF) {
+            const int v = j % V, jj = j / V;
+            const int k = jj & (NS - 1);
+            if constexpr (NS > 1) apply_tw<R>(x[i], T1[k], T4[k]);
+            dft<R>(x[i]);
+            const int base = (jj / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[lidx<V, PAD>(v, base + r * NS)] = x[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int V, bool PAD, int P, int NS>
+__device__ __forceinline__ void fft_passes(float2* buf, const float2* T1, const float2* T4) {
+    constexpr int LOGL = ilog2c(L);
+    if constexpr (P < npasses(LOGL) && L > 1) {
+        constexpr int R = 1 << pass_bits(LOGL, P);
+        constexpr int off = tw_offset(LOGL, P);
+        stockham_pass<L, V, PAD, R, NS>(buf, T1 + off, T4 + off);
+        fft_passes<L, V, PAD, P + 1, NS * R>(buf, T1, T4);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void load_costab(float* dst, const float* cos_tables) {
+    constexpr int n = L / 4 + 1;
+    const int off = costab_offset(L);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = cos_tables[off + i];
 }
 
 // Series sample i of a trial as FFT input: x[i] - mean32 in f32 for FP32
@@ -142,7 +290,7 @@ __device__ __forceinline__ void store_power(void* spec, int fp16, size_t idx, fl
         ((float*)spec)[idx] = p;
 }
 
-__device__ double block_sum(double v, double* red) {
+__device__ double block_sum(double v, double* red) {  // red: blockDim/64 entries
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) red[w] = v;
@@ -153,38 +301,56 @@ __device__ double block_sum(double v, double* red) {
     return s;
 }
 
+// Power of X[k] from Z[k] and Z[M-k] (zp) with W = W_N^k:
+//   X = (Z + conj Zp)/2 + W (Z - conj Zp)/(2i)
+__device__ __forceinline__ float r2c_power(float2 zk, float2 zp, float2 w) {
+    const float2 e = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
+    const float2 od = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
+    const float2 x = cadd(e, cmul(w, od));
+    const float xx = x.x * x.x, yy = x.y * x.y;
+    return xx + yy;
+}
+
 // ---------------------------------------------------------- single pass
-__global__ __launch_bounds__(256) void k_fft_single(const void* __restrict__ series, int fp16,
+template <int L>
+__global__ __launch_bounds__(512) void k_fft_single(const void* __restrict__ series, int fp16,
                                                     const float* __restrict__ mean32,
                                                     const float* __restrict__ corr16, uint32_t n,
-                                                    uint32_t series_pitch, uint32_t M, int logM,
+                                                    uint32_t series_pitch,
                                                     void* __restrict__ spec, uint64_t spec_pitch,
                                                     double* __restrict__ partial,
-                                                    const float2* __restrict__ tw, TwOffsets o) {
-    extern __shared__ float2 buf[];
-    __shared__ double red[4];
+                                                    const float* __restrict__ cos_tables) {
+    constexpr int M = L;
+    constexpr int NBUF = M + (M >> 4) + 1;
+    constexpr int NT = tw_total(M) + 1;
+    extern __shared__ float2 smem[];
+    float2* buf = smem;
+    float2* T1 = smem + NBUF;                // pass twiddles W^k
+    float2* T4 = T1 + NT;                    // pass twiddles W^(4k)
+    const float* CN = cos_tables + costab_offset(2 * M);  // length-2M table (r2c twiddles)
+    __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.x;
     const size_t row = (size_t)trial * series_pitch;
     const float mean = fp16 ? 0.f : mean32[trial];
     const float corr = fp16 ? corr16[trial] : 0.f;
-    for (uint32_t m = threadIdx.x; m < M; m += blockDim.x)
-        buf[m] = load_pair(series, fp16, row, 2 * m, n, mean, corr);
+    if constexpr (M >= 4) build_pass_tw<M>(T1, T4, cos_tables);
+    for (int m = threadIdx.x; m < M; m += blockDim.x)
+        buf[lidx<1, true>(0, m)] = load_pair(series, fp16, row, 2u * m, n, mean, corr);
     __syncthreads();
-    lds_fft<4>(buf, (int)M, logM, 1, tw + o.p1);
+    fft_passes<M, 1, true, 0, 1>(buf, T1, T4);
     const float inv_n = 1.0f / (float)n;
     double acc = 0.0;
     const size_t srow = (size_t)trial * spec_pitch;
-    for (uint32_t k = threadIdx.x; k <= M; k += blockDim.x) {
-        const float2 zk = buf[k & (M - 1)], zp = buf[(M - k) & (M - 1)];
-        const float2 e = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
-        const float2 od = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
-        float2 x;
-        if (k == M)
-            x = csub(e, od);
-        else
-            x = cadd(e, cmul(tw[o.rlo + k], od));
-        const float xx = x.x * x.x, yy = x.y * x.y;
-        const float p = xx + yy;
+    for (int k = threadIdx.x; k <= M; k += blockDim.x) {
+        const float2 zk = buf[lidx<1, true>(0, k & (M - 1))];
+        const float2 zp = buf[lidx<1, true>(0, (M - k) & (M - 1))];
+        float p;
+        if (k == M) {  // Nyquist: X[M] = E[0] - O[0]
+            const float xm = zk.x - zk.y;
+            p = xm * xm;
+        } else {
+            p = r2c_power(zk, zp, twiddle<2 * M>(CN, k));
+        }
         store_power(spec, fp16, srow + k, p, inv_n);
         acc += fp16 ? (double)(p * inv_n) : (double)p;
     }
@@ -193,77 +359,91 @@ __global__ __launch_bounds__(256) void k_fft_single(const void* __restrict__ ser
 }
 
 // --------------------------------------------------------------- pass 1
-__global__ __launch_bounds__(256) void k_fft_pass1(const void* __restrict__ series, int fp16,
+template <int M2>
+__global__ __launch_bounds__(512) void k_fft_pass1(const void* __restrict__ series, int fp16,
                                                    const float* __restrict__ mean32,
                                                    const float* __restrict__ corr16, uint32_t n,
                                                    uint32_t series_pitch, uint32_t M, uint32_t m1,
-                                                   uint32_t m2, int log_m2, uint32_t C,
                                                    float2* __restrict__ work,
-                                                   const float2* __restrict__ tw, TwOffsets o) {
-    extern __shared__ float2 buf[];  // [C][m2]
+                                                   const float2* __restrict__ tw4,
+                                                   const float* __restrict__ cos_tables) {
+    constexpr int C = 8192 / M2;  // columns per workgroup
+    constexpr int NT = tw_total(M2) + 1;
+    extern __shared__ float2 smem[];
+    float2* buf = smem;  // [M2][C], element r of column col at r*C + col
+    float2* T1 = smem + C * M2;
+    float2* T4 = T1 + NT;
     const uint32_t trial = blockIdx.y;
     const uint32_t c0 = blockIdx.x * C;
     const size_t row = (size_t)trial * series_pitch;
     const float mean = fp16 ? 0.f : mean32[trial];
     const float corr = fp16 ? corr16[trial] : 0.f;
-    for (uint32_t idx = threadIdx.x; idx < C * m2; idx += blockDim.x) {
-        const uint32_t r = idx / C, col = idx - r * C;
-        const uint32_t i = 2u * (c0 + col + m1 * r);
-        buf[col * m2 + r] = load_pair(series, fp16, row, i, n, mean, corr);
+    build_pass_tw<M2>(T1, T4, cos_tables);
+    // rows r with 2*(c0 + m1*r) >= n hold only padding zeros
+    for (int idx = threadIdx.x; idx < C * M2; idx += FFT_THREADS) {
+        const int r = idx / C, col = idx % C;
+        buf[idx] = load_pair(series, fp16, row, 2u * (c0 + col + m1 * r), n, mean, corr);
     }
     __syncthreads();
-    lds_fft<8>(buf, (int)m2, log_m2, (int)C, tw + o.p1);
+    fft_passes<M2, C, false, 0, 1>(buf, T1, T4);
     float2* w = work + (size_t)trial * M;
-    for (uint32_t idx = threadIdx.x; idx < C * m2; idx += blockDim.x) {
-        const uint32_t k2 = idx / C, col = idx - k2 * C;
+    const float2* tlo = tw4;
+    const float2* thi = tw4 + 2048;
+    for (int idx = threadIdx.x; idx < C * M2; idx += FFT_THREADS) {
+        const int k2 = idx / C, col = idx % C;
         const uint32_t a = (uint32_t)(((uint64_t)(c0 + col) * k2) & (M - 1));
-        const float2 t = cmul(tw[o.hi + (a >> 11)], tw[o.lo + (a & 2047)]);
-        w[(size_t)k2 * m1 + c0 + col] = cmul(buf[col * m2 + k2], t);
+        const float2 t = cmul(thi[a >> 11], tlo[a & 2047]);
+        w[(size_t)k2 * m1 + c0 + col] = cmul(buf[idx], t);
     }
 }
 
 // --------------------------------------------------------------- pass 2
-__global__ __launch_bounds__(256) void k_fft_pass2(const float2* __restrict__ work, int fp16,
-                                                   uint32_t n, uint32_t M, uint32_t m1,
-                                                   int log_m1, uint32_t m2,
+template <int M1>
+__global__ __launch_bounds__(512) void k_fft_pass2(const float2* __restrict__ work, int fp16,
+                                                   uint32_t n, uint32_t M, uint32_t m2,
                                                    void* __restrict__ spec, uint64_t spec_pitch,
                                                    double* __restrict__ partial,
-                                                   const float2* __restrict__ tw, TwOffsets o) {
-    extern __shared__ float2 buf[];  // [2][m1]
-    __shared__ double red[4];
+                                                   const float2* __restrict__ twrow,
+                                                   const float* __restrict__ cos_tables) {
+    constexpr int NBUF = 2 * M1 + ((2 * M1) >> 4) + 1;
+    constexpr int NT = tw_total(M1) + 1;
+    extern __shared__ float2 smem[];
+    float2* buf = smem;                    // 2 rows interleaved, padded
+    float2* T1 = smem + NBUF;
+    float2* T4 = T1 + NT;
+    const float* C2 = cos_tables + costab_offset(2 * M1);  // W_N^(m2 k1) = W_(2 M1)^k1
+    __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.y;
     const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
     const float2* w = work + (size_t)trial * M;
-    for (uint32_t k1 = threadIdx.x; k1 < m1; k1 += blockDim.x) {
-        buf[k1] = w[(size_t)r1 * m1 + k1];
-        buf[m1 + k1] = w[(size_t)r2 * m1 + k1];
+    if constexpr (M1 >= 4) build_pass_tw<M1>(T1, T4, cos_tables);
+    for (int k1 = threadIdx.x; k1 < M1; k1 += FFT_THREADS) {
+        buf[lidx<2, true>(0, k1)] = w[(size_t)r1 * M1 + k1];
+        buf[lidx<2, true>(1, k1)] = w[(size_t)r2 * M1 + k1];
     }
     __syncthreads();
-    lds_fft<8>(buf, (int)m1, log_m1, 2, tw + o.p2);
+    fft_passes<M1, 2, true, 0, 1>(buf, T1, T4);
     const float inv_n = 1.0f / (float)n;
     const size_t srow = (size_t)trial * spec_pitch;
     double acc = 0.0;
-    const uint32_t nsel = r1 == r2 ? 1u : 2u;
-    for (uint32_t idx = threadIdx.x; idx < nsel * m1; idx += blockDim.x) {
-        const uint32_t sel = idx / m1, k1 = idx - sel * m1;
+    const int nsel = r1 == r2 ? 1 : 2;
+    const float2 wr1 = twrow[r1], wr2 = twrow[r2];
+    for (int idx = threadIdx.x; idx < nsel * M1; idx += FFT_THREADS) {
+        const int sel = idx / M1, k1 = idx % M1;
         const uint32_t rowk = sel ? r2 : r1;
-        const float2 zk = buf[sel * m1 + k1];
+        const float2 zk = buf[lidx<2, true>(sel, k1)];
         float2 zp;
         if (rowk == 0)
-            zp = buf[(m1 - k1) & (m1 - 1)];
+            zp = buf[lidx<2, true>(0, (M1 - k1) & (M1 - 1))];
         else
-            zp = buf[(nsel == 1 ? 0u : (1u - sel)) * m1 + (m1 - 1 - k1)];
-        const float2 e = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
-        const float2 od = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
-        const float2 wk = cmul(tw[o.rlo + rowk], tw[o.rhi + k1]);
-        const float2 x = cadd(e, cmul(wk, od));
-        const float xx = x.x * x.x, yy = x.y * x.y;
-        const float p = xx + yy;
-        store_power(spec, fp16, srow + (size_t)rowk * m1 + k1, p, inv_n);
+            zp = buf[lidx<2, true>(nsel == 1 ? 0 : 1 - sel, M1 - 1 - k1)];
+        const float2 wk = cmul(sel ? wr2 : wr1, twiddle<2 * M1>(C2, k1));
+        const float p = r2c_power(zk, zp, wk);
+        store_power(spec, fp16, srow + (size_t)rowk * M1 + k1, p, inv_n);
         acc += fp16 ? (double)(p * inv_n) : (double)p;
-        if (rowk == 0 && k1 == 0) {  // Nyquist bin k = M
-            const float2 xm = csub(e, od);
-            const float pm = xm.x * xm.x + xm.y * xm.y;
+        if (rowk == 0 && k1 == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
+            const float xm = zk.x - zk.y;
+            const float pm = xm * xm;
             store_power(spec, fp16, srow + M, pm, inv_n);
             acc += fp16 ? (double)(pm * inv_n) : (double)pm;
         }
@@ -272,32 +452,72 @@ __global__ __launch_bounds__(256) void k_fft_pass2(const float2* __restrict__ wo
     if (threadIdx.x == 0) partial[(size_t)trial * (m2 / 2 + 1) + r1] = s;
 }
 
-static int ilog2(uint32_t v) {
-    int l = 0;
-    while ((1u << l) < v) l++;
-    return l;
+// ------------------------------------------------------------- launchers
+template <typename K>
+static void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <int L>
+static void launch_single(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
+                          const float* mean32, const float* corr16, uint32_t ntrials, void* spec,
+                          double* partial, const float* cos_tables) {
+    const size_t shm = (size_t)(L + (L >> 4) + 1 + 2 * (tw_total(L) + 1)) * sizeof(float2);
+    allow_lds(k_fft_single<L>, shm);
+    hipLaunchKernelGGL(k_fft_single<L>, dim3(ntrials), dim3(FFT_THREADS), shm, s, series, fp16, mean32,
+                       corr16, g.n, g.series_pitch, spec, g.spec_pitch, partial, cos_tables);
+}
+
+template <int M1>
+static void launch_pass2(hipStream_t s, const SpecGeom& g, int fp16, uint32_t ntrials,
+                         const float2* work, void* spec, double* partial, const float2* twrow,
+                         const float* cos_tables) {
+    const size_t shm = (size_t)(2 * M1 + ((2 * M1) >> 4) + 1 + 2 * (tw_total(M1) + 1)) * sizeof(float2);
+    allow_lds(k_fft_pass2<M1>, shm);
+    hipLaunchKernelGGL(k_fft_pass2<M1>, dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), shm, s, work, fp16,
+                       g.n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
+}
+
+template <int M2>
+static void launch_pass1(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
+                         const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
+                         const float2* tw4, const float* cos_tables) {
+    constexpr int C = 8192 / M2;
+    const size_t shm = (size_t)(C * M2 + 2 * (tw_total(M2) + 1)) * sizeof(float2);
+    allow_lds(k_fft_pass1<M2>, shm);
+    hipLaunchKernelGGL(k_fft_pass1<M2>, dim3(g.m1 / C, ntrials), dim3(FFT_THREADS), shm, s, series, fp16,
+                       mean32, corr16, g.n, g.series_pitch, g.m, g.m1, work, tw4, cos_tables);
 }
 
 void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
                 const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
-                void* spec, double* partial, const float2* tw) {
-    const TwOffsets o = tw_offsets(g);
+                void* spec, double* partial, const float2* tw, const float* cos_tables) {
     if (g.m1 == 1) {
-        const size_t shm = (size_t)g.m * sizeof(float2);
-        hipLaunchKernelGGL(k_fft_single, dim3(ntrials), dim3(256), shm, s, series, fp16, mean32,
-                           corr16, g.n, g.series_pitch, g.m, ilog2(g.m), spec, g.spec_pitch,
-                           partial, tw, o);
+        switch (g.m) {
+#define SINGLE(L) \
+    case L: launch_single<L>(s, g, series, fp16, mean32, corr16, ntrials, spec, partial, cos_tables); break;
+            SINGLE(2) SINGLE(4) SINGLE(8) SINGLE(16) SINGLE(32) SINGLE(64) SINGLE(128)
+            SINGLE(256) SINGLE(512) SINGLE(1024) SINGLE(2048) SINGLE(4096)
+#undef SINGLE
+        }
         return;
     }
-    uint32_t C = 8192 / g.m2;
-    if (C > g.m1) C = g.m1;
-    if (C < 1) C = 1;
-    hipLaunchKernelGGL(k_fft_pass1, dim3(g.m1 / C, ntrials), dim3(256),
-                       (size_t)C * g.m2 * sizeof(float2), s, series, fp16, mean32, corr16, g.n,
-                       g.series_pitch, g.m, g.m1, g.m2, ilog2(g.m2), C, work, tw, o);
-    hipLaunchKernelGGL(k_fft_pass2, dim3(g.m2 / 2 + 1, ntrials), dim3(256),
-                       (size_t)2 * g.m1 * sizeof(float2), s, work, fp16, g.n, g.m, g.m1,
-                       ilog2(g.m1), g.m2, spec, g.spec_pitch, partial, tw, o);
+    // tw layout: [W_M^j, j<2048][W_M^(2048 j), j<M/2048][W_N^k2, k2<m2]
+    const float2* tw4 = tw;
+    const float2* twrow = tw + 2048 + g.m / 2048;
+    if (g.m2 == 512)
+        launch_pass1<512>(s, g, series, fp16, mean32, corr16, ntrials, work, tw4, cos_tables);
+    else
+        launch_pass1<1024>(s, g, series, fp16, mean32, corr16, ntrials, work, tw4, cos_tables);
+    switch (g.m1) {
+#define PASS2(L) \
+    case L: launch_pass2<L>(s, g, fp16, ntrials, work, spec, partial, twrow, cos_tables); break;
+        PASS2(8) PASS2(16) PASS2(32) PASS2(64) PASS2(128) PASS2(256) PASS2(512) PASS2(1024)
+        PASS2(2048) PASS2(4096)
+#undef PASS2
+    }
 }
 
 }  // namespace psr

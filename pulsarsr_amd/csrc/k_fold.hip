@@ -55,24 +55,31 @@ void launch_fold(hipStream_t s, const float* series, uint64_t n, double tsamp,
 
 // mean32[t] = (sum/n) as f32 (FP32 path); corr16[t] = (m0 - sum/n) * inv_s0
 // (FP16 path: the offset that turns stored (x - m0)/s0 into (x - mean)/s0).
-// The trial sum is reduced from the dedispersion kernel's per-tile partials in
-// tile order.
-__global__ void k_trial_mean(const double* __restrict__ partial, uint32_t ntiles,
-                             uint32_t ntrials, uint32_t n, double m0, double inv_s0,
-                             float* __restrict__ mean32, float* __restrict__ corr16) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per trial reduces the dedispersion kernel's per-tile partials in a
+// fixed order (lane-strided sums, then a fixed butterfly), so the result is
+// deterministic; for integer-valued series every partial is an exact integer.
+__global__ __launch_bounds__(256) void k_trial_mean(const double* __restrict__ partial,
+                                                    uint32_t ntiles, uint32_t ntrials, uint32_t n,
+                                                    double m0, double inv_s0,
+                                                    float* __restrict__ mean32,
+                                                    float* __restrict__ corr16) {
+    const uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (t >= ntrials) return;
     double s = 0.0;
-    for (uint32_t i = 0; i < ntiles; i++) s += partial[(size_t)t * ntiles + i];
-    const double mean = s / (double)n;
-    mean32[t] = (float)mean;
-    corr16[t] = (float)((m0 - mean) * inv_s0);
+    for (uint32_t i = lane; i < ntiles; i += 64) s += partial[(size_t)t * ntiles + i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) {
+        const double mean = s / (double)n;
+        mean32[t] = (float)mean;
+        corr16[t] = (float)((m0 - mean) * inv_s0);
+    }
 }
 
 void launch_trial_mean(hipStream_t s, const double* partial, uint32_t ntiles, uint32_t ntrials,
                        uint32_t n, double m0, double inv_s0, float* mean32, float* corr16) {
-    hipLaunchKernelGGL(k_trial_mean, dim3((ntrials + 255) / 256), dim3(256), 0, s, partial,
-                       ntiles, ntrials, n, m0, inv_s0, mean32, corr16);
+    hipLaunchKernelGGL(k_trial_mean, dim3((ntrials + 3) / 4), dim3(256), 0, s, partial, ntiles,
+                       ntrials, n, m0, inv_s0, mean32, corr16);
 }
 
 }  // namespace psr

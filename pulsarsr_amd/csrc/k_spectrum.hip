@@ -46,6 +46,12 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
                                                 float* __restrict__ scales, uint32_t cw) {
     extern __shared__ uint32_t tile[];
     __shared__ float s_floor;
+    // tile position p is stored at p + p / m2: the gather below writes positions
+    // m2 apart from neighbouring lanes, which the padding spreads over the banks
+    // (64-bin blocks never straddle a pad, as m2 >= 512 is a multiple of 64).
+    // The natural single-column layout (m1 == 1) needs no padding.
+    const int64_t padu = m1 == 1 ? ((int64_t)1 << 40) : (int64_t)m2;
+    auto P = [padu](int64_t p) { return p + p / padu; };
     const uint32_t trial = blockIdx.y;
     const uint32_t k1a = blockIdx.x * cw;
     const uint32_t k1b = min(k1a + cw, m1);
@@ -56,7 +62,7 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
     const uint32_t TL = nt_blocks * 64;
     const size_t srow = (size_t)trial * spec_pitch;
 
-    for (uint32_t i = threadIdx.x; i < TL; i += blockDim.x) tile[i] = KEY_INF;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)P(TL) + 1; i += blockDim.x) tile[i] = KEY_INF;
     if (threadIdx.x == 0) {
         double S = 0.0;
         for (uint32_t i = 0; i < npart; i++) S += partial[(size_t)trial * npart + i];
@@ -73,15 +79,15 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
         const int64_t k = (int64_t)k2 + (int64_t)m2 * k1;
         const int64_t pos = k - tile_lo;
         if (pos < 0 || pos >= (int64_t)TL || k < 1) continue;
-        tile[pos] = load_key(spec_t, fp16, srow + (size_t)k2 * m1 + (size_t)k1);
+        tile[P(pos)] = load_key(spec_t, fp16, srow + (size_t)k2 * m1 + (size_t)k1);
     }
-    if (B1 == nblocks && threadIdx.x == 0) tile[(int64_t)m - tile_lo] = load_key(spec_t, fp16, srow + m);
+    if (B1 == nblocks && threadIdx.x == 0) tile[P((int64_t)m - tile_lo)] = load_key(spec_t, fp16, srow + m);
     __syncthreads();
     // natural-order raw spectrum for the workgroup's own blocks (bin 0 included)
     {
         const uint32_t kb = B0 * 64, ke = min(B1 * 64, nbins);
         for (uint32_t k = kb + threadIdx.x; k < ke; k += blockDim.x) {
-            uint32_t key = tile[(int64_t)k - tile_lo];
+            uint32_t key = tile[P((int64_t)k - tile_lo)];
             if (k == 0) key = load_key(spec_t, fp16, srow);
             if (fp16)
                 ((uint16_t*)raw_nat)[srow + k] = (uint16_t)key;
@@ -92,7 +98,8 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
     // sort each 64-bin block ascending (bin 0 and absent bins hold KEY_INF)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (uint32_t b = wv; b < nt_blocks; b += blockDim.x >> 6) {
-        uint32_t key = tile[b * 64 + lane];
+        uint32_t* blk = tile + P((int64_t)b * 64);
+        uint32_t key = blk[lane];
 #pragma unroll
         for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
@@ -103,7 +110,7 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
                 key = (lower == up) ? min(key, other) : max(key, other);
             }
         }
-        tile[b * 64 + lane] = key;
+        blk[lane] = key;
     }
     __syncthreads();
     // one thread per block of this workgroup: exact window median by cut search
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
             cnt = s1 > s0 ? (int)(s1 - s0) : 0;
         }
         nn[j] = cnt;
-        A[j] = tile + (b - B0 + j) * 64;
+        A[j] = tile + P((int64_t)(b - B0 + j) * 64);
         p[j] = cnt >> 1;
         len += cnt;
     }
@@ -202,7 +209,8 @@ void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp1
     // one workgroup's blocks must fit the 256 threads that select them
     while (cw > 1 && (cw * g.m2) / 64 + 1 > 256) cw >>= 1;
     const uint32_t blocks = (cw * g.m2) / 64 + 1 + 4;
-    const size_t shm = (size_t)blocks * 64 * sizeof(uint32_t);
+    const size_t pads = g.m1 == 1 ? 0 : (size_t)blocks * 64 / g.m2;
+    const size_t shm = ((size_t)blocks * 64 + pads + 1) * sizeof(uint32_t);
     dim3 grid((g.m1 + cw - 1) / cw, ntrials);
     hipLaunchKernelGGL(k_whiten, grid, dim3(256), shm, s, spec_t, fp16, g.m, g.m1, g.m2, g.nbins,
                        g.nblocks, g.spec_pitch, partial, npartials, raw_nat, scales, cw);
