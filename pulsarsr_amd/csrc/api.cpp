@@ -564,11 +564,10 @@ static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const doub
     const size_t es = fp16 ? 2 : 4;
     const uint32_t n = g.n;
     int rc;
-    const bool int_path = o->kind == KIND_U8;
-    const uint32_t ntiles = int_path ? (n + 511) / 512 : (n + 255) / 256;
+    const uint32_t ntiles_u8 = (n + 511) / 512, ntiles_f32 = (n + 255) / 256;
     if ((rc = ensure(c, c->delays, (size_t)nt * o->nchans * sizeof(int32_t))) ||
         (rc = ensure(c, c->series, (size_t)nt * g.series_pitch * es + 64)) ||
-        (rc = ensure(c, c->dd_partial, (size_t)nt * ntiles * sizeof(double))) ||
+        (rc = ensure(c, c->dd_partial, (size_t)nt * ntiles_f32 * sizeof(double))) ||
         (rc = ensure(c, c->mean32, (size_t)nt * sizeof(float))) ||
         (rc = ensure(c, c->corr16, (size_t)nt * sizeof(float))))
         return rc;
@@ -578,18 +577,21 @@ static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const doub
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
     const double inv_s0 = 1.0 / o->s0;
-    if (int_path) {
+    bool int_done = false;
+    if (o->kind == KIND_U8) {
         const uint32_t spread = spread_bound(o, dms_host, nt, 32);
         if ((rc = ensure(c, c->dd_table, dedisp_u8_table_words(o->nchans, nt) * sizeof(int32_t)))) return rc;
-        launch_dedisp_u8(s, (const uint8_t*)o->data, o->pitch, o->nchans, o->maxval,
-                         (const int32_t*)c->delays.p, (int32_t*)c->dd_table.p, nt, n, c->series.p,
-                         g.series_pitch, fp16, (float)o->m0, (float)inv_s0, (double*)c->dd_partial.p,
-                         spread);
-    } else {
+        int_done = launch_dedisp_u8(s, (const uint8_t*)o->data, o->pitch, o->nchans, o->maxval,
+                                    (const int32_t*)c->delays.p, (int32_t*)c->dd_table.p, nt, n,
+                                    c->series.p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
+                                    (double*)c->dd_partial.p, spread);
+    }
+    if (!int_done) {
         launch_dedisp_f32(s, o->data, o->kind, o->pitch, o->nchans, (const int32_t*)c->delays.p, nt,
                           n, c->series.p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
                           (double*)c->dd_partial.p);
     }
+    const uint32_t ntiles = int_done ? ntiles_u8 : ntiles_f32;
     HIP_TRY(hipGetLastError());
     launch_trial_mean(s, (const double*)c->dd_partial.p, ntiles, nt, n, o->m0, inv_s0,
                       (float*)c->mean32.p, (float*)c->corr16.p);

@@ -66,7 +66,9 @@ void launch_chan_stats(hipStream_t s, const void* data, int kind, uint32_t nchan
 void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double ref_inv,
                    double tsamp, const double* dms, uint32_t ntrials, int32_t* out);
 // integer dedispersion (u8 rows), outputs f32 exact sums or FP16 scaled
-void launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
+// Returns false (and launches nothing) when the block's delay spread needs more
+// LDS than a workgroup has; the caller then uses launch_dedisp_f32.
+bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
                       double* partial, uint32_t spread_max);

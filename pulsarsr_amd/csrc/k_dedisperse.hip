@@ -6,6 +6,8 @@
 // any order; the kernel accumulates them with packed 16-bit adds and widens to
 // 32 bits before any half-word can overflow.  16-bit and float samples use a
 // kernel that adds in channel order in f32, exactly as the reference does.
+#include <algorithm>
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -139,32 +141,42 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 
 // ------------------------------------------------- integer dedispersion (u8)
 // Workgroup = 4 waves sharing one time tile of DD_TILE = 512 outputs and a block
-// of DD_TRIALS consecutive DM trials (each wave owns DD_M trials).  Channels are
-// staged chunk by chunk into LDS as 32-bit words
-//     w[i] = s[base + i] | s[base + i + 128] << 16,
-// so any delay offset is an aligned dword address.  Lane l reads the adjacent
-// words e + 256g + 2l and e + 256g + 2l + 1 (g = 0, 1) with one ds_read2_b32 per
-// g -- the 64 lanes of a wave touch 128 distinct consecutive dwords, free of
-// bank conflicts -- and accumulates each word (two outputs, 128 apart) with a
-// single v_pk_add_u16.  Lane l owns outputs 256g + 2l + {0, 1, 128, 129}.
-// The per-trial LDS byte offsets come from a table built by k_dd_offsets and
-// are read with scalar loads.
+// of DD_TRIALS = 32 consecutive (ascending) DM trials; each wave owns DD_M = 8
+// trials.  Channels are staged chunk by chunk into LDS as 32-bit words
+//     w[i] = s[base + i] | s[base + i + 256] << 16,
+// with base = the block's smallest delay of the channel rounded down to 4.  A
+// trial whose delay is base + e needs words e .. e + 255; lane l takes words
+// e + 4l .. e + 4l + 3 with one 16-byte ds_read_b128 and adds them with four
+// v_pk_add_u16 (8 outputs: 4l + j and 256 + 4l + j).  ds_read_b128 needs a
+// 16-byte aligned address, so every channel row is staged four times, copy k
+// starting k words later; the trial reads copy e & 3 at word e & ~3.  The 64
+// lanes of a wave read 1 KiB of consecutive LDS, free of bank conflicts.
+//
+// Per-trial LDS addresses are staged with the rows (no scalar loads inside the
+// channel loop, so LDS waits stay fine-grained).  16-bit accumulators widen to
+// 32 bits every `phase_chans` channels, before a half-word can overflow.
+//
+// Workgroups are numbered XCD-aware: consecutive workgroup ids go to different
+// XCDs, so id is remapped so that each XCD walks a contiguous run of
+// (tile, block) work items with the block index fastest -- the channel data of
+// a tile is then fetched into that XCD's L2 once and reused by all DM blocks.
 constexpr int DD_M = 8;
 constexpr int DD_TILE = 512;
 constexpr int DD_TRIALS = 4 * DD_M;
+constexpr int DD_XCDS = 8;
 
 __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
     u16x2 x = __builtin_bit_cast(u16x2, a), y = __builtin_bit_cast(u16x2, b);
     return __builtin_bit_cast(uint32_t, x + y);
 }
 
-// bases[blk][c] = delay of the block's first trial rounded down to 4 samples;
-// offs[blk][wave][c][m] = 4 * (delay of trial wave*DD_M + m - bases[blk][c]),
-// the LDS byte offset; a wave's offsets of consecutive channels are contiguous.
-__global__ __launch_bounds__(256) void k_dd_offsets(const int32_t* __restrict__ delays,
-                                                    uint32_t nchans, uint32_t ntrials,
-                                                    int32_t* __restrict__ bases,
-                                                    int32_t* __restrict__ offs) {
+// bases[blk][c] = delay of the block's first trial at channel c, rounded down to
+// a multiple of 4; rel[blk][c][m] = delay of trial blk*32 + m minus that base
+// (trials past the end repeat the last trial).
+__global__ __launch_bounds__(256) void k_dd_table(const int32_t* __restrict__ delays,
+                                                  uint32_t nchans, uint32_t ntrials,
+                                                  int32_t* __restrict__ bases,
+                                                  int32_t* __restrict__ rel) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
     if (i >= (uint64_t)nblk * nchans) return;
@@ -174,79 +186,100 @@ __global__ __launch_bounds__(256) void k_dd_offsets(const int32_t* __restrict__ 
     bases[i] = base;
     for (int m = 0; m < DD_TRIALS; m++) {
         const uint32_t d = min(d0 + m, ntrials - 1);
-        const uint32_t wv = m / DD_M, mm = m % DD_M;
-        offs[(((size_t)blk * 4 + wv) * nchans + c) * DD_M + mm] = 4 * (delays[(size_t)d * nchans + c] - base);
+        rel[i * DD_TRIALS + m] = delays[(size_t)d * nchans + c] - base;
     }
+}
+
+__device__ __forceinline__ uint32_t load_u32(const uint8_t* row, uint64_t off, uint64_t pitch) {
+    return off + 4 <= pitch ? *(const uint32_t*)(row + off) : 0u;
 }
 
 __global__ __launch_bounds__(256) void k_dedisp_u8(
     const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_chans,
-    const int32_t* __restrict__ bases, const int32_t* __restrict__ offs, uint32_t ntrials,
-    uint32_t n, void* __restrict__ out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
-    double* __restrict__ partial, uint32_t W, uint32_t C) {
-    extern __shared__ uint32_t lds[];  // [C][W] words
-    const uint32_t tile = blockIdx.x;
+    const int32_t* __restrict__ bases, const int32_t* __restrict__ rel, uint32_t ntrials,
+    uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out, uint32_t out_pitch,
+    int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t W4, uint32_t C) {
+    extern __shared__ __align__(16) uint32_t lds[];  // [C][4][W4] words, then [C][32] addresses
+    int32_t* laddr = (int32_t*)(lds + (size_t)C * 4 * W4);
+
+    // XCD-aware work index (bijective over total = ntiles * nblk)
+    const uint32_t total = ntiles * nblk;
+    const uint32_t id = blockIdx.x;
+    const uint32_t xcd = id % DD_XCDS, local = id / DD_XCDS;
+    const uint32_t q = total / DD_XCDS, r = total % DD_XCDS;
+    const uint32_t work = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
+    const uint32_t tile = work / nblk, blk = work % nblk;
+
     const uint32_t t0 = tile * DD_TILE;
-    const uint32_t blk = blockIdx.y;
     const uint32_t d0 = blk * DD_TRIALS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int32_t* bblk = bases + (size_t)blk * nchans;
-    const int32_t* oblk = offs + ((size_t)blk * 4 + wave) * nchans * DD_M;
+    const int32_t* rblk = rel + (size_t)blk * nchans * DD_TRIALS;
 
-    // acc16[m][q]: word q of trial m (q = 2g + j: outputs 256g + 2l + j and +128)
     uint32_t acc16[DD_M][4];
     uint32_t acc32[DD_M][8];
 #pragma unroll
     for (int m = 0; m < DD_M; m++) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc16[m][q] = 0;
+        for (int j = 0; j < 4; j++) acc16[m][j] = 0;
 #pragma unroll
-        for (int q = 0; q < 8; q++) acc32[m][q] = 0;
+        for (int j = 0; j < 8; j++) acc32[m][j] = 0;
     }
     uint32_t in_phase = 0;
-    const uint32_t quads = W / 4;
+    const uint32_t quads = W4 / 4;
+    const char* lds_lane = (const char*)lds + 16 * lane;
     for (uint32_t cc = 0; cc < nchans; cc += C) {
         const uint32_t nc = min(C, nchans - cc);
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nc * quads; i += blockDim.x) {
-            const uint32_t c = i / quads, q = i - c * quads;
-            const uint32_t base = (uint32_t)bblk[cc + c];
+            const uint32_t c = i / quads, qd = i - c * quads;
             const uint8_t* row = data + (size_t)(cc + c) * pitch;
-            const uint64_t off = (uint64_t)t0 + base + 4u * q;
-            const uint32_t a = off + 4 <= pitch ? *(const uint32_t*)(row + off) : 0u;
-            const uint32_t b = off + 132 <= pitch ? *(const uint32_t*)(row + off + 128) : 0u;
-            uint4 w;
-            w.x = __builtin_amdgcn_perm(b, a, 0x0c040c00u);
-            w.y = __builtin_amdgcn_perm(b, a, 0x0c050c01u);
-            w.z = __builtin_amdgcn_perm(b, a, 0x0c060c02u);
-            w.w = __builtin_amdgcn_perm(b, a, 0x0c070c03u);
-            *(uint4*)(lds + (size_t)c * W + 4 * q) = w;
+            const uint64_t off = (uint64_t)t0 + (uint32_t)bblk[cc + c] + 4u * qd;
+            const uint32_t a0 = load_u32(row, off, pitch), a1 = load_u32(row, off + 4, pitch);
+            const uint32_t b0 = load_u32(row, off + 256, pitch), b1 = load_u32(row, off + 260, pitch);
+            uint32_t* dst = lds + (size_t)c * 4 * W4 + 4 * qd;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                // bytes k..k+3 of the 8-byte windows starting at off and off + 256
+                const uint32_t a = __builtin_amdgcn_alignbyte(a1, a0, k);
+                const uint32_t b = __builtin_amdgcn_alignbyte(b1, b0, k);
+                uint4 w;
+                w.x = __builtin_amdgcn_perm(b, a, 0x0c040c00u);
+                w.y = __builtin_amdgcn_perm(b, a, 0x0c050c01u);
+                w.z = __builtin_amdgcn_perm(b, a, 0x0c060c02u);
+                w.w = __builtin_amdgcn_perm(b, a, 0x0c070c03u);
+                *(uint4*)(dst + (size_t)k * W4) = w;
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < nc * DD_TRIALS; i += blockDim.x) {
+            const uint32_t c = i / DD_TRIALS;
+            const int32_t e = rblk[(size_t)cc * DD_TRIALS + i];
+            laddr[i] = (int32_t)((((size_t)c * 4 + (e & 3)) * W4 + (uint32_t)(e & ~3)) * 4);
         }
         __syncthreads();
-        const char* rows = (const char*)lds + 8 * lane;
 #pragma unroll 2
         for (uint32_t c = 0; c < nc; c++) {
-            const int32_t* oc = oblk + (size_t)(cc + c) * DD_M;
-            const char* rowc = rows + (size_t)c * W * 4;
+            const int4* ap = (const int4*)(laddr + c * DD_TRIALS + wave * DD_M);
+            const int4 A0 = ap[0], A1 = ap[1];
+            const int addr[DD_M] = {A0.x, A0.y, A0.z, A0.w, A1.x, A1.y, A1.z, A1.w};
 #pragma unroll
             for (int m = 0; m < DD_M; m++) {
-                const uint32_t* r0 = (const uint32_t*)(rowc + oc[m]);
-                const uint32_t* r1 = (const uint32_t*)(rowc + oc[m] + 1024);
-                acc16[m][0] = pk_add(acc16[m][0], r0[0]);
-                acc16[m][1] = pk_add(acc16[m][1], r0[1]);
-                acc16[m][2] = pk_add(acc16[m][2], r1[0]);
-                acc16[m][3] = pk_add(acc16[m][3], r1[1]);
+                const uint4 v = *(const uint4*)(lds_lane + addr[m]);
+                acc16[m][0] = pk_add(acc16[m][0], v.x);
+                acc16[m][1] = pk_add(acc16[m][1], v.y);
+                acc16[m][2] = pk_add(acc16[m][2], v.z);
+                acc16[m][3] = pk_add(acc16[m][3], v.w);
             }
             if (++in_phase == phase_chans) {
                 in_phase = 0;
 #pragma unroll
                 for (int m = 0; m < DD_M; m++)
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        acc32[m][2 * q] += acc16[m][q] & 0xffffu;
-                        acc32[m][2 * q + 1] += acc16[m][q] >> 16;
-                        acc16[m][q] = 0;
+                    for (int j = 0; j < 4; j++) {
+                        acc32[m][j] += acc16[m][j] & 0xffffu;
+                        acc32[m][4 + j] += acc16[m][j] >> 16;
+                        acc16[m][j] = 0;
                     }
             }
         }
@@ -254,11 +287,10 @@ __global__ __launch_bounds__(256) void k_dedisp_u8(
 #pragma unroll
     for (int m = 0; m < DD_M; m++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            acc32[m][2 * q] += acc16[m][q] & 0xffffu;
-            acc32[m][2 * q + 1] += acc16[m][q] >> 16;
+        for (int j = 0; j < 4; j++) {
+            acc32[m][j] += acc16[m][j] & 0xffffu;
+            acc32[m][4 + j] += acc16[m][j] >> 16;
         }
-    const uint32_t ntiles = gridDim.x;
     const int m0i = (int)m0;
 #pragma unroll
     for (int m = 0; m < DD_M; m++) {
@@ -266,20 +298,35 @@ __global__ __launch_bounds__(256) void k_dedisp_u8(
         double lsum = 0.0;
         if (d < ntrials) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int g = q >> 1, j = q & 1;
-#pragma unroll
-                for (int hh = 0; hh < 2; hh++) {
-                    const uint32_t t = t0 + 256 * g + 2 * lane + j + 128 * hh;
-                    const uint32_t v = acc32[m][2 * q + hh];
-                    if (t < n) {
-                        lsum += (double)v;
-                        if (fp16)
-                            ((__half*)out)[(size_t)d * out_pitch + t] =
-                                __float2half_rn((float)((int)v - m0i) * inv_s0);
-                        else
-                            ((float*)out)[(size_t)d * out_pitch + t] = (float)v;
+            for (int hh = 0; hh < 2; hh++) {
+                const uint32_t t = t0 + 256 * hh + 4 * lane;  // outputs t .. t + 3
+                const uint32_t* v = &acc32[m][4 * hh];
+                const size_t o = (size_t)d * out_pitch + t;
+                if (t + 3 < n) {
+                    lsum += (double)v[0] + (double)v[1] + (double)v[2] + (double)v[3];
+                    if (fp16) {
+                        __half2 h01 = __floats2half2_rn((float)((int)v[0] - m0i) * inv_s0,
+                                                        (float)((int)v[1] - m0i) * inv_s0);
+                        __half2 h23 = __floats2half2_rn((float)((int)v[2] - m0i) * inv_s0,
+                                                        (float)((int)v[3] - m0i) * inv_s0);
+                        uint2 pk;
+                        pk.x = __builtin_bit_cast(uint32_t, h01);
+                        pk.y = __builtin_bit_cast(uint32_t, h23);
+                        *(uint2*)((__half*)out + o) = pk;
+                    } else {
+                        *(float4*)((float*)out + o) =
+                            make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
                     }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (t + j < n) {
+                            lsum += (double)v[j];
+                            if (fp16)
+                                ((__half*)out)[o + j] = __float2half_rn((float)((int)v[j] - m0i) * inv_s0);
+                            else
+                                ((float*)out)[o + j] = (float)v[j];
+                        }
                 }
             }
         }
@@ -295,32 +342,40 @@ size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
     return nblk * nchans * (1 + DD_TRIALS);
 }
 
-void launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
+bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
                       double* partial, uint32_t spread_max) {
+    // a trial reads words (e & ~3) .. (e & ~3) + 255 with e <= spread_max + 3
+    const uint64_t W4 = ((uint64_t)spread_max + 3 + 3) / 4 * 4 + DD_TILE / 2;
+    const uint64_t per_chan = 16 * W4 + 4 * DD_TRIALS;  // four row copies + addresses
+    uint64_t budget = 40 * 1024;
+    if (per_chan > budget) budget = 150 * 1024;
+    if (per_chan > budget) return false;
+    uint32_t C = (uint32_t)std::min<uint64_t>(budget / per_chan, 64);
+    C = std::min(C, nchans);
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
     int32_t* bases = table;
-    int32_t* offs = table + nblk * nchans;
-    hipLaunchKernelGGL(k_dd_offsets, dim3((unsigned)((nblk * nchans + 255) / 256)), dim3(256), 0, s,
-                       delays, nchans, ntrials, bases, offs);
-    // rows must cover the largest in-block delay spread, 3 bytes of alignment
-    // slack, and the footprint of a trial's reads (words 0..383 past its offset).
-    uint32_t W = spread_max + 3 + 384 + 1;
-    W = (W + 3) & ~3u;
-    const uint32_t budget = 40 * 1024;
-    uint32_t C = 64;
-    while (C > 1 && (size_t)C * W * 4 > budget) C >>= 1;
+    int32_t* rel = table + nblk * nchans;
+    hipLaunchKernelGGL(k_dd_table, dim3((unsigned)((nblk * nchans + 255) / 256)), dim3(256), 0, s,
+                       delays, nchans, ntrials, bases, rel);
     uint32_t phase = 65535u / (maxval ? maxval : 1);
     phase = phase > 4096 ? 4096 : phase;
-    const size_t shm = (size_t)C * W * 4;
-    dim3 grid((n + DD_TILE - 1) / DD_TILE, (unsigned)nblk);
-    hipLaunchKernelGGL(k_dedisp_u8, grid, dim3(256), shm, s, data, pitch, nchans, phase, bases, offs,
-                       ntrials, n, out, out_pitch, fp16, m0, inv_s0, partial, W, C);
+    const size_t shm = (size_t)C * per_chan;
+    if (shm > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dedisp_u8),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    const uint32_t ntiles = (n + DD_TILE - 1) / DD_TILE;
+    const uint64_t total = (uint64_t)ntiles * nblk;
+    hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(256), shm, s, data, pitch, nchans,
+                       phase, bases, rel, ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16,
+                       m0, inv_s0, partial, (uint32_t)W4, C);
+    return true;
 }
 
 // ---------------------------------------- channel-ordered f32 dedispersion
-// For 16-bit and float samples: one thread per output, f32 adds in channel
+// For 16-bit and float samples (and 8-bit ones whose delay spread is too wide
+// for the LDS-staged kernel): one thread per output, f32 adds in channel
 // order starting from 0.0 -- the reference's exact operation sequence.
 __global__ __launch_bounds__(256) void k_dedisp_f32(const void* __restrict__ data, int kind,
                                                     uint64_t pitch, uint32_t nchans,
@@ -336,8 +391,9 @@ __global__ __launch_bounds__(256) void k_dedisp_f32(const void* __restrict__ dat
     if (t < n) {
         for (uint32_t c = 0; c < nchans; c++) {
             const uint64_t idx = (uint64_t)c * pitch + t + (uint32_t)dd[c];
-            const float v = kind == KIND_U16 ? (float)((const uint16_t*)data)[idx]
-                                             : ((const float*)data)[idx];
+            const float v = kind == KIND_U8    ? (float)((const uint8_t*)data)[idx]
+                            : kind == KIND_U16 ? (float)((const uint16_t*)data)[idx]
+                                               : ((const float*)data)[idx];
             acc += v;
         }
         if (fp16)
