@@ -40,6 +40,7 @@ struct psr_ctx {
     uint64_t mem_cap = 0;
     uint64_t mem_used = 0;
     DevBuf delays, dd_table, dms, series, dd_partial, mean32, corr16, work, spec_t, raw_nat, scales,
+        blk_sorted, floors,
         fft_partial, peaks, peak_counts, cands, counts, twiddles, costabs, birdies, fold_series, fold_per,
         fold_off, fold_sums, fold_counts;
     SpecGeom tw_geom{};
@@ -234,10 +235,13 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
     const uint32_t npart = g.m1 == 1 ? 1u : g.m2 / 2 + 1;
     int rc;
     if ((rc = upload_twiddles(c, g))) return rc;
-    if ((rc = ensure(c, c->work, g.m1 == 1 ? 8 : (size_t)nt * g.m * sizeof(float2))) ||
+    const size_t work_bytes = g.m1 == 1 ? 8 : (size_t)fft_work_trials(g, nt) * g.m * sizeof(float2);
+    if ((rc = ensure(c, c->work, work_bytes)) ||
         (rc = ensure(c, c->spec_t, (size_t)nt * g.spec_pitch * es)) ||
         (rc = ensure(c, c->raw_nat, (size_t)nt * g.spec_pitch * es)) ||
         (rc = ensure(c, c->scales, (size_t)nt * g.nblocks * sizeof(float))) ||
+        (rc = ensure(c, c->blk_sorted, (size_t)nt * g.spec_pitch * es)) ||
+        (rc = ensure(c, c->floors, (size_t)nt * sizeof(float))) ||
         (rc = ensure(c, c->fft_partial, (size_t)nt * npart * sizeof(double))) ||
         (rc = ensure(c, c->peaks, (size_t)nt * PEAK_CAP * sizeof(RawPeak))) ||
         (rc = ensure(c, c->peak_counts, (size_t)nt * sizeof(uint32_t))) ||
@@ -257,7 +261,7 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
     launch_whiten(s, g, c->spec_t.p, (int)p.fp16, nt, (const double*)c->fft_partial.p, npart,
-                  c->raw_nat.p, (float*)c->scales.p);
+                  c->raw_nat.p, c->blk_sorted.p, (float*)c->floors.p, (float*)c->scales.p);
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
     launch_harmsum(s, g, p, c->raw_nat.p, (const float*)c->scales.p, bh.data(), nt,
@@ -323,9 +327,10 @@ void psr_ctx_destroy(psr_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf* all[] = {&c->delays, &c->dd_table, &c->dms, &c->series, &c->dd_partial, &c->mean32, &c->corr16,
-                     &c->work, &c->spec_t, &c->raw_nat, &c->scales, &c->fft_partial, &c->peaks,
-                     &c->peak_counts, &c->cands, &c->counts, &c->twiddles, &c->costabs, &c->birdies,
+    DevBuf* all[] = {&c->delays, &c->dd_table, &c->dms, &c->series, &c->dd_partial, &c->mean32,
+                     &c->corr16, &c->work, &c->spec_t, &c->raw_nat, &c->scales, &c->blk_sorted,
+                     &c->floors, &c->fft_partial, &c->peaks, &c->peak_counts, &c->cands,
+                     &c->counts, &c->twiddles, &c->costabs, &c->birdies,
                      &c->fold_series, &c->fold_per, &c->fold_off, &c->fold_sums, &c->fold_counts};
     for (DevBuf* b : all) release(c, *b);
     for (int i = 0; i < 8; i++)
@@ -667,8 +672,8 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
     for (size_t i = 0; i < ntrials; i++) sorted[i] = dms[order[i]];
     // trials per chunk: keep the chunk's working set near 16 GiB
     const size_t es = p.fp16 ? 2 : 4;
-    const size_t per_trial = (size_t)g.series_pitch * es + (g.m1 == 1 ? 0 : (size_t)g.m * 8) +
-                             2 * (size_t)g.spec_pitch * es + (size_t)PEAK_CAP * sizeof(RawPeak) +
+    const size_t per_trial = (size_t)g.series_pitch * es +
+                             3 * (size_t)g.spec_pitch * es + (size_t)PEAK_CAP * sizeof(RawPeak) +
                              cap_per_trial * sizeof(CandidateOut) + (size_t)o->nchans * 4 + 4096;
     size_t chunk = std::max<size_t>(32, ((size_t)16 << 30) / per_trial);
     chunk = std::min(chunk, ntrials);

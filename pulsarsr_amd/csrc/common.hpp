@@ -91,10 +91,14 @@ __host__ __device__ constexpr int costab_offset(int L) {
 }
 constexpr int COSTAB_SIZE = costab_offset(2 * COSTAB_MAX);
 void fill_costabs(float* host_table);
-// whitening scales + natural-order raw spectrum
+// Trials per four-step FFT batch (the work buffer holds this many trials).
+constexpr uint64_t FFT_BATCH_BYTES = 1ull << 30;
+uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials);
+// whitening scales + natural-order raw spectrum; `sorted` (spectrum-sized) and
+// `floors` (one float per trial) are scratch
 void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp16,
                    uint32_t ntrials, const double* partial_sums, uint32_t npartials,
-                   void* raw_nat, float* scales);
+                   void* raw_nat, void* sorted, float* floors, float* scales);
 // harmonic sums + peaks
 void launch_harmsum(hipStream_t s, const SpecGeom& g, const SearchParams& p, const void* raw_nat,
                     const float* scales, const uint32_t* birdies, uint32_t ntrials,

@@ -257,37 +257,41 @@ __device__ __forceinline__ void fft_passes(float2* buf, const float2* T1, const 
     }
 }
 
-template <int L>
-__device__ __forceinline__ void load_costab(float* dst, const float* cos_tables) {
-    constexpr int n = L / 4 + 1;
-    const int off = costab_offset(L);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = cos_tables[off + i];
-}
-
 // Series sample i of a trial as FFT input: x[i] - mean32 in f32 for FP32
-// storage, h[i] + corr (already scaled) for FP16 storage, zero beyond n.
-__device__ __forceinline__ float2 load_pair(const void* series, int fp16, size_t row,
-                                            uint32_t i, uint32_t n, float mean, float corr) {
-    float2 z = make_float2(0.f, 0.f);
-    if (i >= n) return z;
-    if (fp16) {
-        const __half2 h = *(const __half2*)((const __half*)series + row + i);
-        const float2 f = __half22float2(h);
-        z.x = f.x + corr;
-        z.y = i + 1 < n ? f.y + corr : 0.f;
+// storage, h[i] + corr (already scaled) for FP16 storage, zero beyond n.  The
+// second element of a pair may sit one past the row's end (inside the pitch or
+// the buffer's slack) and is then discarded.
+template <bool F16>
+__device__ __forceinline__ float2 load_pair(const void* series, size_t row, uint32_t i, uint32_t n,
+                                            float mean, float corr) {
+    // unconditional load from a valid address (so unrolled loads can be
+    // batched), then select
+    const uint32_t is = i < n ? i : 0u;
+    float2 f;
+    if (F16) {
+        f = __half22float2(*(const __half2*)((const __half*)series + row + is));
+        f.x += corr;
+        f.y += corr;
     } else {
-        const float2 f = *(const float2*)((const float*)series + row + i);
-        z.x = f.x - mean;
-        z.y = i + 1 < n ? f.y - mean : 0.f;
+        f = *(const float2*)((const float*)series + row + is);
+        f.x -= mean;
+        f.y -= mean;
     }
-    return z;
+    return make_float2(i < n ? f.x : 0.f, i + 1 < n ? f.y : 0.f);
 }
 
-__device__ __forceinline__ void store_power(void* spec, int fp16, size_t idx, float p, float inv_n) {
-    if (fp16)
-        ((__half*)spec)[idx] = __float2half_rn(sqrtf(p * inv_n));
-    else
-        ((float*)spec)[idx] = p;
+// Stored spectrum value: the power (FP32) or the amplitude sqrt(power / n)
+// (FP16, which keeps every value in half range); returns what the whitening
+// floor sums.
+template <bool F16>
+__device__ __forceinline__ double store_power(void* spec, size_t idx, float p, float inv_n) {
+    if (F16) {
+        const float q = p * inv_n;
+        ((__half*)spec)[idx] = __float2half_rn(sqrtf(q));
+        return (double)q;
+    }
+    ((float*)spec)[idx] = p;
+    return (double)p;
 }
 
 __device__ double block_sum(double v, double* red) {  // red: blockDim/64 entries
@@ -311,9 +315,15 @@ __device__ __forceinline__ float r2c_power(float2 zk, float2 zp, float2 w) {
     return xx + yy;
 }
 
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Global loads of a kernel's input are issued together (fully unrolled into
+// registers) before the twiddle tables are built and the data is written to
+// LDS, so their latencies overlap instead of adding up.
+
 // ---------------------------------------------------------- single pass
-template <int L>
-__global__ __launch_bounds__(512) void k_fft_single(const void* __restrict__ series, int fp16,
+template <int L, bool F16>
+__global__ __launch_bounds__(512, 4) void k_fft_single(const void* __restrict__ series,
                                                     const float* __restrict__ mean32,
                                                     const float* __restrict__ corr16, uint32_t n,
                                                     uint32_t series_pitch,
@@ -323,6 +333,7 @@ __global__ __launch_bounds__(512) void k_fft_single(const void* __restrict__ ser
     constexpr int M = L;
     constexpr int NBUF = M + (M >> 4) + 1;
     constexpr int NT = tw_total(M) + 1;
+    constexpr int PER = cdiv(M, FFT_THREADS);
     extern __shared__ float2 smem[];
     float2* buf = smem;
     float2* T1 = smem + NBUF;                // pass twiddles W^k
@@ -331,17 +342,29 @@ __global__ __launch_bounds__(512) void k_fft_single(const void* __restrict__ ser
     __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.x;
     const size_t row = (size_t)trial * series_pitch;
-    const float mean = fp16 ? 0.f : mean32[trial];
-    const float corr = fp16 ? corr16[trial] : 0.f;
+    const float mean = F16 ? 0.f : mean32[trial];
+    const float corr = F16 ? corr16[trial] : 0.f;
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int mm = threadIdx.x + i * FFT_THREADS;
+        v[i] = mm < M ? load_pair<F16>(series, row, 2u * mm, n, mean, corr) : make_float2(0.f, 0.f);
+    }
     if constexpr (M >= 4) build_pass_tw<M>(T1, T4, cos_tables);
-    for (int m = threadIdx.x; m < M; m += blockDim.x)
-        buf[lidx<1, true>(0, m)] = load_pair(series, fp16, row, 2u * m, n, mean, corr);
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int mm = threadIdx.x + i * FFT_THREADS;
+        if (mm < M) buf[lidx<1, true>(0, mm)] = v[i];
+    }
     __syncthreads();
     fft_passes<M, 1, true, 0, 1>(buf, T1, T4);
     const float inv_n = 1.0f / (float)n;
     double acc = 0.0;
     const size_t srow = (size_t)trial * spec_pitch;
-    for (int k = threadIdx.x; k <= M; k += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < cdiv(M + 1, FFT_THREADS); i++) {
+        const int k = threadIdx.x + i * FFT_THREADS;
+        if (k > M) break;
         const float2 zk = buf[lidx<1, true>(0, k & (M - 1))];
         const float2 zp = buf[lidx<1, true>(0, (M - k) & (M - 1))];
         float p;
@@ -351,16 +374,15 @@ __global__ __launch_bounds__(512) void k_fft_single(const void* __restrict__ ser
         } else {
             p = r2c_power(zk, zp, twiddle<2 * M>(CN, k));
         }
-        store_power(spec, fp16, srow + k, p, inv_n);
-        acc += fp16 ? (double)(p * inv_n) : (double)p;
+        acc += store_power<F16>(spec, srow + k, p, inv_n);
     }
     const double s = block_sum(acc, red);
     if (threadIdx.x == 0) partial[trial] = s;
 }
 
 // --------------------------------------------------------------- pass 1
-template <int M2>
-__global__ __launch_bounds__(512) void k_fft_pass1(const void* __restrict__ series, int fp16,
+template <int M2, bool F16>
+__global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ series,
                                                    const float* __restrict__ mean32,
                                                    const float* __restrict__ corr16, uint32_t n,
                                                    uint32_t series_pitch, uint32_t M, uint32_t m1,
@@ -369,6 +391,7 @@ __global__ __launch_bounds__(512) void k_fft_pass1(const void* __restrict__ seri
                                                    const float* __restrict__ cos_tables) {
     constexpr int C = 8192 / M2;  // columns per workgroup
     constexpr int NT = tw_total(M2) + 1;
+    constexpr int PER = C * M2 / FFT_THREADS;
     extern __shared__ float2 smem[];
     float2* buf = smem;  // [M2][C], element r of column col at r*C + col
     float2* T1 = smem + C * M2;
@@ -376,37 +399,54 @@ __global__ __launch_bounds__(512) void k_fft_pass1(const void* __restrict__ seri
     const uint32_t trial = blockIdx.y;
     const uint32_t c0 = blockIdx.x * C;
     const size_t row = (size_t)trial * series_pitch;
-    const float mean = fp16 ? 0.f : mean32[trial];
-    const float corr = fp16 ? corr16[trial] : 0.f;
-    build_pass_tw<M2>(T1, T4, cos_tables);
+    const float mean = F16 ? 0.f : mean32[trial];
+    const float corr = F16 ? corr16[trial] : 0.f;
     // rows r with 2*(c0 + m1*r) >= n hold only padding zeros
-    for (int idx = threadIdx.x; idx < C * M2; idx += FFT_THREADS) {
+    float2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int idx = threadIdx.x + i * FFT_THREADS;
         const int r = idx / C, col = idx % C;
-        buf[idx] = load_pair(series, fp16, row, 2u * (c0 + col + m1 * r), n, mean, corr);
+        v[i] = load_pair<F16>(series, row, 2u * (c0 + col + m1 * r), n, mean, corr);
     }
+    build_pass_tw<M2>(T1, T4, cos_tables);
+#pragma unroll
+    for (int i = 0; i < PER; i++) buf[threadIdx.x + i * FFT_THREADS] = v[i];
     __syncthreads();
     fft_passes<M2, C, false, 0, 1>(buf, T1, T4);
     float2* w = work + (size_t)trial * M;
     const float2* tlo = tw4;
     const float2* thi = tw4 + 2048;
-    for (int idx = threadIdx.x; idx < C * M2; idx += FFT_THREADS) {
-        const int k2 = idx / C, col = idx % C;
-        const uint32_t a = (uint32_t)(((uint64_t)(c0 + col) * k2) & (M - 1));
-        const float2 t = cmul(thi[a >> 11], tlo[a & 2047]);
-        w[(size_t)k2 * m1 + c0 + col] = cmul(buf[idx], t);
+    // thread's column is fixed (FFT_THREADS is a multiple of C); k2 steps by
+    // FFT_THREADS / C
+    const int col = threadIdx.x % C, k20 = threadIdx.x / C;
+    constexpr int KS = FFT_THREADS / C;
+    float2 tl[PER], th[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const uint32_t a = (uint32_t)(((uint64_t)(c0 + col) * (k20 + KS * i)) & (M - 1));
+        tl[i] = tlo[a & 2047];
+        th[i] = thi[a >> 11];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int k2 = k20 + KS * i;
+        w[(size_t)k2 * m1 + c0 + col] = cmul(buf[k2 * C + col], cmul(th[i], tl[i]));
     }
 }
 
 // --------------------------------------------------------------- pass 2
-template <int M1>
-__global__ __launch_bounds__(512) void k_fft_pass2(const float2* __restrict__ work, int fp16,
-                                                   uint32_t n, uint32_t M, uint32_t m2,
+template <int M1, bool F16>
+__global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__ work, uint32_t n,
+                                                   uint32_t M, uint32_t m2,
                                                    void* __restrict__ spec, uint64_t spec_pitch,
                                                    double* __restrict__ partial,
                                                    const float2* __restrict__ twrow,
                                                    const float* __restrict__ cos_tables) {
     constexpr int NBUF = 2 * M1 + ((2 * M1) >> 4) + 1;
     constexpr int NT = tw_total(M1) + 1;
+    constexpr int PER = cdiv(M1, FFT_THREADS);
+    constexpr int PER2 = cdiv(2 * M1, FFT_THREADS);
     extern __shared__ float2 smem[];
     float2* buf = smem;                    // 2 rows interleaved, padded
     float2* T1 = smem + NBUF;
@@ -416,10 +456,23 @@ __global__ __launch_bounds__(512) void k_fft_pass2(const float2* __restrict__ wo
     const uint32_t trial = blockIdx.y;
     const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
     const float2* w = work + (size_t)trial * M;
+    float2 va[PER], vb[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int k1 = threadIdx.x + i * FFT_THREADS;
+        if (M1 % FFT_THREADS == 0 || k1 < M1) {
+            va[i] = w[(size_t)r1 * M1 + k1];
+            vb[i] = w[(size_t)r2 * M1 + k1];
+        }
+    }
     if constexpr (M1 >= 4) build_pass_tw<M1>(T1, T4, cos_tables);
-    for (int k1 = threadIdx.x; k1 < M1; k1 += FFT_THREADS) {
-        buf[lidx<2, true>(0, k1)] = w[(size_t)r1 * M1 + k1];
-        buf[lidx<2, true>(1, k1)] = w[(size_t)r2 * M1 + k1];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int k1 = threadIdx.x + i * FFT_THREADS;
+        if (M1 % FFT_THREADS == 0 || k1 < M1) {
+            buf[lidx<2, true>(0, k1)] = va[i];
+            buf[lidx<2, true>(1, k1)] = vb[i];
+        }
     }
     __syncthreads();
     fft_passes<M1, 2, true, 0, 1>(buf, T1, T4);
@@ -428,7 +481,16 @@ __global__ __launch_bounds__(512) void k_fft_pass2(const float2* __restrict__ wo
     double acc = 0.0;
     const int nsel = r1 == r2 ? 1 : 2;
     const float2 wr1 = twrow[r1], wr2 = twrow[r2];
-    for (int idx = threadIdx.x; idx < nsel * M1; idx += FFT_THREADS) {
+    float2 ck[PER2];
+#pragma unroll
+    for (int i = 0; i < PER2; i++) {
+        const int idx = threadIdx.x + i * FFT_THREADS;
+        ck[i] = idx < nsel * M1 ? twiddle<2 * M1>(C2, idx % M1) : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < PER2; i++) {
+        const int idx = threadIdx.x + i * FFT_THREADS;
+        if (idx >= nsel * M1) break;
         const int sel = idx / M1, k1 = idx % M1;
         const uint32_t rowk = sel ? r2 : r1;
         const float2 zk = buf[lidx<2, true>(sel, k1)];
@@ -437,15 +499,12 @@ __global__ __launch_bounds__(512) void k_fft_pass2(const float2* __restrict__ wo
             zp = buf[lidx<2, true>(0, (M1 - k1) & (M1 - 1))];
         else
             zp = buf[lidx<2, true>(nsel == 1 ? 0 : 1 - sel, M1 - 1 - k1)];
-        const float2 wk = cmul(sel ? wr2 : wr1, twiddle<2 * M1>(C2, k1));
+        const float2 wk = cmul(sel ? wr2 : wr1, ck[i]);
         const float p = r2c_power(zk, zp, wk);
-        store_power(spec, fp16, srow + (size_t)rowk * M1 + k1, p, inv_n);
-        acc += fp16 ? (double)(p * inv_n) : (double)p;
+        acc += store_power<F16>(spec, srow + (size_t)rowk * M1 + k1, p, inv_n);
         if (rowk == 0 && k1 == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
             const float xm = zk.x - zk.y;
-            const float pm = xm * xm;
-            store_power(spec, fp16, srow + M, pm, inv_n);
-            acc += fp16 ? (double)(pm * inv_n) : (double)pm;
+            acc += store_power<F16>(spec, srow + M, xm * xm, inv_n);
         }
     }
     const double s = block_sum(acc, red);
@@ -460,44 +519,48 @@ static void allow_lds(K kernel, size_t bytes) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <int L>
-static void launch_single(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
+template <int L, bool F16>
+static void launch_single(hipStream_t s, const SpecGeom& g, const void* series,
                           const float* mean32, const float* corr16, uint32_t ntrials, void* spec,
                           double* partial, const float* cos_tables) {
     const size_t shm = (size_t)(L + (L >> 4) + 1 + 2 * (tw_total(L) + 1)) * sizeof(float2);
-    allow_lds(k_fft_single<L>, shm);
-    hipLaunchKernelGGL(k_fft_single<L>, dim3(ntrials), dim3(FFT_THREADS), shm, s, series, fp16, mean32,
-                       corr16, g.n, g.series_pitch, spec, g.spec_pitch, partial, cos_tables);
+    allow_lds(k_fft_single<L, F16>, shm);
+    hipLaunchKernelGGL((k_fft_single<L, F16>), dim3(ntrials), dim3(FFT_THREADS), shm, s, series,
+                       mean32, corr16, g.n, g.series_pitch, spec, g.spec_pitch, partial, cos_tables);
 }
 
-template <int M1>
-static void launch_pass2(hipStream_t s, const SpecGeom& g, int fp16, uint32_t ntrials,
-                         const float2* work, void* spec, double* partial, const float2* twrow,
+template <int M1, bool F16>
+static void launch_pass2(hipStream_t s, const SpecGeom& g, uint32_t ntrials, const float2* work,
+                         void* spec, double* partial, const float2* twrow, const float* cos_tables) {
+    const size_t shm =
+        (size_t)(2 * M1 + ((2 * M1) >> 4) + 1 + 2 * (tw_total(M1) + 1)) * sizeof(float2);
+    allow_lds(k_fft_pass2<M1, F16>, shm);
+    hipLaunchKernelGGL((k_fft_pass2<M1, F16>), dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), shm,
+                       s, work, g.n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
+}
+
+template <int M2, bool F16>
+static void launch_pass1(hipStream_t s, const SpecGeom& g, const void* series, const float* mean32,
+                         const float* corr16, uint32_t ntrials, float2* work, const float2* tw4,
                          const float* cos_tables) {
-    const size_t shm = (size_t)(2 * M1 + ((2 * M1) >> 4) + 1 + 2 * (tw_total(M1) + 1)) * sizeof(float2);
-    allow_lds(k_fft_pass2<M1>, shm);
-    hipLaunchKernelGGL(k_fft_pass2<M1>, dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), shm, s, work, fp16,
-                       g.n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
-}
-
-template <int M2>
-static void launch_pass1(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
-                         const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
-                         const float2* tw4, const float* cos_tables) {
     constexpr int C = 8192 / M2;
     const size_t shm = (size_t)(C * M2 + 2 * (tw_total(M2) + 1)) * sizeof(float2);
-    allow_lds(k_fft_pass1<M2>, shm);
-    hipLaunchKernelGGL(k_fft_pass1<M2>, dim3(g.m1 / C, ntrials), dim3(FFT_THREADS), shm, s, series, fp16,
-                       mean32, corr16, g.n, g.series_pitch, g.m, g.m1, work, tw4, cos_tables);
+    allow_lds(k_fft_pass1<M2, F16>, shm);
+    hipLaunchKernelGGL((k_fft_pass1<M2, F16>), dim3(g.m1 / C, ntrials), dim3(FFT_THREADS), shm, s,
+                       series, mean32, corr16, g.n, g.series_pitch, g.m, g.m1, work, tw4,
+                       cos_tables);
 }
 
-void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
-                const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
-                void* spec, double* partial, const float2* tw, const float* cos_tables) {
+template <bool F16>
+static void launch_fft_t(hipStream_t s, const SpecGeom& g, const void* series, const float* mean32,
+                         const float* corr16, uint32_t ntrials, float2* work, void* spec,
+                         double* partial, const float2* tw, const float* cos_tables) {
     if (g.m1 == 1) {
         switch (g.m) {
-#define SINGLE(L) \
-    case L: launch_single<L>(s, g, series, fp16, mean32, corr16, ntrials, spec, partial, cos_tables); break;
+#define SINGLE(L)                                                                             \
+    case L:                                                                                   \
+        launch_single<L, F16>(s, g, series, mean32, corr16, ntrials, spec, partial, cos_tables); \
+        break;
             SINGLE(2) SINGLE(4) SINGLE(8) SINGLE(16) SINGLE(32) SINGLE(64) SINGLE(128)
             SINGLE(256) SINGLE(512) SINGLE(1024) SINGLE(2048) SINGLE(4096)
 #undef SINGLE
@@ -507,17 +570,45 @@ void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
     // tw layout: [W_M^j, j<2048][W_M^(2048 j), j<M/2048][W_N^k2, k2<m2]
     const float2* tw4 = tw;
     const float2* twrow = tw + 2048 + g.m / 2048;
-    if (g.m2 == 512)
-        launch_pass1<512>(s, g, series, fp16, mean32, corr16, ntrials, work, tw4, cos_tables);
-    else
-        launch_pass1<1024>(s, g, series, fp16, mean32, corr16, ntrials, work, tw4, cos_tables);
-    switch (g.m1) {
+    // Trials go through both passes in batches of FFT_BATCH_BYTES of
+    // intermediate, which bounds the work buffer independently of the chunk.
+    const uint32_t tb = fft_work_trials(g, ntrials);
+    const size_t es = F16 ? 2 : 4;
+    const uint32_t npart = g.m2 / 2 + 1;
+    for (uint32_t b = 0; b < ntrials; b += tb) {
+        const uint32_t nb = ntrials - b < tb ? ntrials - b : tb;
+        const void* ser = (const char*)series + (size_t)b * g.series_pitch * es;
+        void* sp = (char*)spec + (size_t)b * g.spec_pitch * es;
+        double* part = partial + (size_t)b * npart;
+        if (g.m2 == 512)
+            launch_pass1<512, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+        else
+            launch_pass1<1024, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+        switch (g.m1) {
 #define PASS2(L) \
-    case L: launch_pass2<L>(s, g, fp16, ntrials, work, spec, partial, twrow, cos_tables); break;
-        PASS2(8) PASS2(16) PASS2(32) PASS2(64) PASS2(128) PASS2(256) PASS2(512) PASS2(1024)
-        PASS2(2048) PASS2(4096)
+    case L: launch_pass2<L, F16>(s, g, nb, work, sp, part, twrow, cos_tables); break;
+            PASS2(8) PASS2(16) PASS2(32) PASS2(64) PASS2(128) PASS2(256) PASS2(512) PASS2(1024)
+            PASS2(2048) PASS2(4096)
 #undef PASS2
+        }
     }
+}
+
+uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials) {
+    if (g.m1 == 1) return ntrials;
+    const uint64_t per = (uint64_t)g.m * sizeof(float2);
+    uint64_t tb = FFT_BATCH_BYTES / per;
+    if (tb < 1) tb = 1;
+    return (uint32_t)(tb < ntrials ? tb : ntrials);
+}
+
+void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
+                const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
+                void* spec, double* partial, const float2* tw, const float* cos_tables) {
+    if (fp16)
+        launch_fft_t<true>(s, g, series, mean32, corr16, ntrials, work, spec, partial, tw, cos_tables);
+    else
+        launch_fft_t<false>(s, g, series, mean32, corr16, ntrials, work, spec, partial, tw, cos_tables);
 }
 
 }  // namespace psr

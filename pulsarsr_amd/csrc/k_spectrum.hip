@@ -4,16 +4,13 @@
 // Whitening (lib.rs:45-60): for 64-bin block b the scale is
 //   max(max(median(raw[lo..hi]) / ln2, floor), 1e-20),  lo = max(64b-128, 1),
 //   hi = min(64b+192, nbins), floor = (mean_f64(raw) * 1e-6) as f32,
-// with the reference's upper median (element len/2 of the sorted window).  Each
-// workgroup loads a run of blocks plus a two-block halo on each side from the
-// (possibly transposed) power spectrum, writes the raw spectrum back in natural
-// order, sorts every 64-bin block in LDS (wave bitonic network), and finds each
-// window's median exactly with a cut search over the five sorted blocks: a
-// position p_j in every block with sum p_j = len/2 such that everything left of
-// the cut is <= everything right of it; the median is then the smallest element
-// right of the cut.  The harmonic-sum kernel applies the whitening on the fly:
-// P_w[i] = raw[i] / scale[i/64] (f32 division, as lib.rs:57-59), 0 at DC and in
-// birdie bins (lib.rs:61-72).
+// with the reference's upper median (element len/2 of the sorted window).  A
+// first kernel reads the (possibly transposed) power spectrum column tile by
+// column tile, writes it back in natural order and sorts every 64-bin block in
+// registers; a second finds each window's median exactly from the five sorted
+// blocks it spans (cut search).  The harmonic-sum kernel applies the whitening
+// on the fly: P_w[i] = raw[i] / scale[i/64] (f32 division, as lib.rs:57-59), 0
+// at DC and in birdie bins (lib.rs:61-72).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -37,85 +34,213 @@ __device__ __forceinline__ float key_power(uint32_t key, int fp16) {
     return __uint_as_float(key);
 }
 
-__global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t, int fp16,
-                                                uint32_t m, uint32_t m1, uint32_t m2,
-                                                uint32_t nbins, uint32_t nblocks,
-                                                uint64_t spec_pitch,
-                                                const double* __restrict__ partial,
-                                                uint32_t npart, void* __restrict__ raw_nat,
-                                                float* __restrict__ scales, uint32_t cw) {
-    extern __shared__ uint32_t tile[];
-    __shared__ float s_floor;
-    // tile position p is stored at p + p / m2: the gather below writes positions
-    // m2 apart from neighbouring lanes, which the padding spreads over the banks
-    // (64-bin blocks never straddle a pad, as m2 >= 512 is a multiple of 64).
-    // The natural single-column layout (m1 == 1) needs no padding.
-    const int64_t padu = m1 == 1 ? ((int64_t)1 << 40) : (int64_t)m2;
-    auto P = [padu](int64_t p) { return p + p / padu; };
+// Ascending bitonic sorting network over registers (indices are compile-time
+// constants once unrolled, so the array stays in VGPRs).
+template <int N>
+__device__ __forceinline__ void bitonic_sort_regs(uint32_t (&v)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = v[i], b = v[l];
+                    if ((i & k) == 0) {
+                        v[i] = min(a, b);
+                        v[l] = max(a, b);
+                    } else {
+                        v[i] = max(a, b);
+                        v[l] = min(a, b);
+                    }
+                }
+            }
+}
+
+constexpr uint32_t SB_BLOCKS = 256;  // natural 64-bin blocks per sorting workgroup (max)
+constexpr uint32_t MB_BLOCKS = 128;  // blocks per median workgroup
+constexpr uint32_t MB_STRIDE = 65;   // LDS words per staged block (odd: spreads banks)
+
+// LDS word of element i of tile block B: 16-byte chunks of a block are XOR
+// permuted so that the 8 lanes of a ds_read_b128 group (8 consecutive blocks)
+// hit different banks, and so that the gather's stores of different columns
+// spread over the banks.
+__device__ __forceinline__ uint32_t sb_h(uint32_t B) { return (B ^ (B >> 3) ^ (B >> 6)) & 7u; }
+__device__ __forceinline__ uint32_t sb_word(uint32_t B, uint32_t i) {
+    return B * 64 + ((((i >> 2) ^ sb_h(B)) << 2) | (i & 3u));
+}
+
+template <bool F16>
+__device__ __forceinline__ void store_keys8(void* dst, size_t o, const uint32_t* k) {
+    // 8 consecutive keys (F16: one 16-byte store; f32: two)
+    if (F16) {
+        *(uint4*)((uint16_t*)dst + o) =
+            make_uint4(k[0] | k[1] << 16, k[2] | k[3] << 16, k[4] | k[5] << 16, k[6] | k[7] << 16);
+    } else {
+        *(uint4*)((uint32_t*)dst + o) = make_uint4(k[0], k[1], k[2], k[3]);
+        *(uint4*)((uint32_t*)dst + o + 4) = make_uint4(k[4], k[5], k[6], k[7]);
+    }
+}
+
+// Stage 1: one workgroup per `cw` columns of the [k2][k1] (k = k2 + m2 k1)
+// spectrum, i.e. the natural bins [m2 k1a, m2 (k1a + cw)).  It writes them back
+// in natural order (raw_nat), and each lane sorts one 64-bin block in registers
+// and stores it ascending to `sorted` (DC and bins beyond Nyquist as KEY_INF).
+// Workgroup 0 of each trial also fixes the whitening floor
+// (mean power * 1e-6, partial sums added in order).  The four-step layout
+// (cw * m2 = SB_BLOCKS * 64) is read with 16-byte row segments, all of a
+// thread's loads issued before any is used; a short single-pass spectrum
+// (one column) takes the scalar path.
+template <bool F16>
+__global__ __launch_bounds__(256) void k_sort_blocks(
+    const void* __restrict__ spec_t, uint32_t m, uint32_t m1, uint32_t m2, uint32_t cw,
+    uint32_t nbins, uint64_t spec_pitch, const double* __restrict__ partial, uint32_t npart,
+    void* __restrict__ raw_nat, void* __restrict__ sorted, float* __restrict__ floors) {
+    extern __shared__ __align__(16) uint32_t tile[];
+    constexpr int E = F16 ? 8 : 4;                            // keys per 16-byte vector
+    constexpr int VPT = (int)(SB_BLOCKS * 64 / E / 256);      // vectors per thread
     const uint32_t trial = blockIdx.y;
     const uint32_t k1a = blockIdx.x * cw;
-    const uint32_t k1b = min(k1a + cw, m1);
-    const uint32_t B0 = (k1a * m2) / 64;
-    const uint32_t B1 = k1b == m1 ? nblocks : (k1b * m2) / 64;
-    const int64_t tile_lo = ((int64_t)B0 - 2) * 64;  // natural bin of tile[0]
-    const uint32_t nt_blocks = (B1 - B0) + 4;
-    const uint32_t TL = nt_blocks * 64;
+    const uint32_t k_lo = k1a * m2;
+    const uint32_t span = cw * m2;             // natural bins of the columns
+    const uint32_t nbk = (span + 63) / 64;     // tile blocks
     const size_t srow = (size_t)trial * spec_pitch;
-
-    for (uint32_t i = threadIdx.x; i < (uint32_t)P(TL) + 1; i += blockDim.x) tile[i] = KEY_INF;
-    if (threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         double S = 0.0;
         for (uint32_t i = 0; i < npart; i++) S += partial[(size_t)trial * npart + i];
-        s_floor = (float)(S / (double)nbins * 1e-6);
+        floors[trial] = (float)(S / (double)nbins * 1e-6);
     }
-    __syncthreads();
-    // gather the tile: columns k1a-1 .. k1b of the [k2][k1] layout, k1 fastest
-    const int64_t c_lo = (int64_t)k1a - 1;
-    const uint32_t ncols = (k1b - k1a) + 2;
-    for (uint32_t idx = threadIdx.x; idx < ncols * m2; idx += blockDim.x) {
-        const uint32_t k2 = idx / ncols;
-        const int64_t k1 = c_lo + (idx - k2 * ncols);
-        if (k1 < 0 || k1 >= (int64_t)m1) continue;
-        const int64_t k = (int64_t)k2 + (int64_t)m2 * k1;
-        const int64_t pos = k - tile_lo;
-        if (pos < 0 || pos >= (int64_t)TL || k < 1) continue;
-        tile[P(pos)] = load_key(spec_t, fp16, srow + (size_t)k2 * m1 + (size_t)k1);
-    }
-    if (B1 == nblocks && threadIdx.x == 0) tile[P((int64_t)m - tile_lo)] = load_key(spec_t, fp16, srow + m);
-    __syncthreads();
-    // natural-order raw spectrum for the workgroup's own blocks (bin 0 included)
-    {
-        const uint32_t kb = B0 * 64, ke = min(B1 * 64, nbins);
-        for (uint32_t k = kb + threadIdx.x; k < ke; k += blockDim.x) {
-            uint32_t key = tile[P((int64_t)k - tile_lo)];
-            if (k == 0) key = load_key(spec_t, fp16, srow);
-            if (fp16)
-                ((uint16_t*)raw_nat)[srow + k] = (uint16_t)key;
-            else
-                ((uint32_t*)raw_nat)[srow + k] = key;
+    if (span == SB_BLOCKS * 64 && cw % E == 0) {
+        const uint32_t vpr = cw / E;  // vectors per row segment
+        uint4 q[VPT];
+#pragma unroll
+        for (int t = 0; t < VPT; t++) {
+            const uint32_t v = threadIdx.x + 256u * t;
+            const uint32_t k2 = v / vpr, qq = v - k2 * vpr;
+            const size_t idx = srow + (size_t)k2 * m1 + k1a + qq * E;
+            q[t] = F16 ? *(const uint4*)((const uint16_t*)spec_t + idx)
+                       : *(const uint4*)((const uint32_t*)spec_t + idx);
         }
-    }
-    // sort each 64-bin block ascending (bin 0 and absent bins hold KEY_INF)
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (uint32_t b = wv; b < nt_blocks; b += blockDim.x >> 6) {
-        uint32_t* blk = tile + P((int64_t)b * 64);
-        uint32_t key = blk[lane];
 #pragma unroll
-        for (int k = 2; k <= 64; k <<= 1) {
+        for (int t = 0; t < VPT; t++) {
+            const uint32_t v = threadIdx.x + 256u * t;
+            const uint32_t k2 = v / vpr, qq = v - k2 * vpr;
+            const uint32_t w[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
 #pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const uint32_t other = __shfl_xor(key, j);
-                const bool up = (lane & k) == 0;
-                const bool lower = (lane & j) == 0;
-                key = (lower == up) ? min(key, other) : max(key, other);
+            for (int e = 0; e < E; e++) {
+                const uint32_t key = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
+                const uint32_t j = k2 + m2 * (qq * E + e);
+                tile[sb_word(j >> 6, j & 63)] = key;
             }
         }
-        blk[lane] = key;
+    } else {
+        for (uint32_t idx = threadIdx.x; idx < span; idx += blockDim.x) {
+            const uint32_t k2 = idx / cw, c = idx - k2 * cw;
+            const uint32_t j = k2 + m2 * c;
+            tile[sb_word(j >> 6, j & 63)] = load_key(spec_t, F16, srow + (size_t)k2 * m1 + k1a + c);
+        }
+        // a short single-pass spectrum (m < 64) shares its only block with Nyquist
+        for (uint32_t j = span + threadIdx.x; j < nbk * 64; j += blockDim.x)
+            tile[sb_word(j >> 6, j & 63)] =
+                k_lo + j == m ? load_key(spec_t, F16, srow + m) : KEY_INF;
     }
     __syncthreads();
-    // one thread per block of this workgroup: exact window median by cut search
-    const uint32_t b = B0 + threadIdx.x;
-    if (b >= B1) return;
+    // natural-order raw spectrum, 8 keys per thread and step
+    const uint32_t jend = min(nbk * 64, m + 1 - k_lo);
+    for (uint32_t j = 8 * threadIdx.x; j < jend; j += 8 * blockDim.x) {
+        const uint32_t Bj = j >> 6, i = j & 63;
+        const uint4 a = *(const uint4*)(tile + Bj * 64 + ((((i >> 2)) ^ sb_h(Bj)) << 2));
+        const uint4 b = *(const uint4*)(tile + Bj * 64 + ((((i >> 2) + 1) ^ sb_h(Bj)) << 2));
+        const uint32_t k[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        if (j + 8 <= jend) {
+            store_keys8<F16>(raw_nat, srow + k_lo + j, k);
+        } else {
+            for (uint32_t e = 0; j + e < jend; e++) {
+                if (F16)
+                    ((uint16_t*)raw_nat)[srow + k_lo + j + e] = (uint16_t)k[e];
+                else
+                    ((uint32_t*)raw_nat)[srow + k_lo + j + e] = k[e];
+            }
+        }
+    }
+    if (k_lo + nbk * 64 <= m && k1a + cw == m1 && threadIdx.x == 0) {
+        const uint32_t key = load_key(spec_t, F16, srow + m);  // Nyquist, outside the tile
+        if (F16)
+            ((uint16_t*)raw_nat)[srow + m] = (uint16_t)key;
+        else
+            ((uint32_t*)raw_nat)[srow + m] = key;
+    }
+    const uint32_t B = threadIdx.x;
+    if (B >= nbk) return;
+    uint32_t v[64];
+    const uint32_t h = sb_h(B);
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        const uint4 q = *(const uint4*)(tile + B * 64 + ((c ^ h) << 2));
+        v[4 * c] = q.x;
+        v[4 * c + 1] = q.y;
+        v[4 * c + 2] = q.z;
+        v[4 * c + 3] = q.w;
+    }
+    if (k_lo == 0 && B == 0) v[0] = KEY_INF;  // DC is not part of any window
+    bitonic_sort_regs<64>(v);
+    const size_t o = srow + (size_t)(k_lo / 64 + B) * 64;
+#pragma unroll
+    for (int c = 0; c < 8; c++) store_keys8<F16>(sorted, o + 8 * c, v + 8 * c);
+}
+
+// Stage 2: one thread per 64-bin block b.  The five sorted blocks b-2 .. b+2
+// of its window are staged in LDS; the exact window median is found by a cut
+// search -- a position p_j in every block with sum p_j = len/2 such that all
+// keys left of the cut are <= all keys right of it; the median is then the
+// smallest key right of the cut.  Block `nsorted` (present when m is a
+// multiple of 64) holds Nyquist alone.
+template <bool F16>
+__global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
+    const void* __restrict__ sorted, const void* __restrict__ spec_t, uint32_t m, uint32_t nbins,
+    uint32_t nblocks, uint64_t spec_pitch, const float* __restrict__ floors,
+    float* __restrict__ scales) {
+    __shared__ uint32_t s[(MB_BLOCKS + 4) * MB_STRIDE];
+    constexpr int E = F16 ? 8 : 4;                                  // keys per 16-byte load
+    constexpr int NV = (int)((MB_BLOCKS + 4) * 64 / E);             // vectors staged
+    constexpr int VPT = (NV + (int)MB_BLOCKS - 1) / (int)MB_BLOCKS;  // per thread
+    const uint32_t trial = blockIdx.y;
+    const uint32_t b0 = blockIdx.x * MB_BLOCKS;
+    const size_t srow = (size_t)trial * spec_pitch;
+    const uint32_t nsorted = (m + 63) / 64;
+    uint4 q[VPT];
+#pragma unroll
+    for (int t = 0; t < VPT; t++) {
+        const int v = threadIdx.x + MB_BLOCKS * t;
+        const int64_t gb = (int64_t)b0 - 2 + (v * E) / 64;
+        q[t] = make_uint4(KEY_INF, KEY_INF, KEY_INF, KEY_INF);
+        if (v < NV && gb >= 0 && gb < (int64_t)nsorted) {
+            const size_t idx = srow + (size_t)gb * 64 + (v * E) % 64;
+            q[t] = F16 ? *(const uint4*)((const uint16_t*)sorted + idx)
+                       : *(const uint4*)((const uint32_t*)sorted + idx);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < VPT; t++) {
+        const int v = threadIdx.x + MB_BLOCKS * t;
+        if (v >= NV) break;
+        const uint32_t lb = (v * E) / 64, i0 = (v * E) % 64;
+        const int64_t gb = (int64_t)b0 - 2 + lb;
+        const uint32_t w[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            uint32_t key = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
+            if (!(gb >= 0 && gb < (int64_t)nsorted)) key = KEY_INF;
+            // block `nsorted` (m a multiple of 64) holds Nyquist alone
+            if (gb == (int64_t)nsorted && gb < (int64_t)nblocks && i0 + e == 0)
+                key = load_key(spec_t, F16, srow + m);
+            s[lb * MB_STRIDE + i0 + e] = key;
+        }
+    }
+    __syncthreads();
+    const uint32_t b = b0 + threadIdx.x;
+    if (b >= nblocks) return;
     int nn[5];
     uint32_t L[5], U[5];
     int p[5];
@@ -131,7 +256,7 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
             cnt = s1 > s0 ? (int)(s1 - s0) : 0;
         }
         nn[j] = cnt;
-        A[j] = tile + P((int64_t)(b - B0 + j) * 64);
+        A[j] = s + (threadIdx.x + j) * MB_STRIDE;
         p[j] = cnt >> 1;
         len += cnt;
     }
@@ -195,25 +320,33 @@ __global__ __launch_bounds__(256) void k_whiten(const void* __restrict__ spec_t,
     uint32_t umin = U[0];
 #pragma unroll
     for (int j = 1; j < 5; j++) umin = min(umin, U[j]);
-    float med = len > 0 ? key_power(umin - 1u, fp16) : 0.0f;
+    float med = len > 0 ? key_power(umin - 1u, F16) : 0.0f;
     if (len == 0) med = 0.0f;  // rfi::median of an empty slice is 0.0
     const float LN2 = 0.693147180559945309417232121458176568f;
-    const float scale = fmaxf(fmaxf(med / LN2, s_floor), 1e-20f);
+    const float scale = fmaxf(fmaxf(med / LN2, floors[trial]), 1e-20f);
     scales[(size_t)trial * nblocks + b] = scale;
 }
 
 void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp16,
                    uint32_t ntrials, const double* partial, uint32_t npartials, void* raw_nat,
-                   float* scales) {
-    uint32_t cw = g.m1 == 1 ? 1 : (g.m1 < 16 ? g.m1 : 16);
-    // one workgroup's blocks must fit the 256 threads that select them
-    while (cw > 1 && (cw * g.m2) / 64 + 1 > 256) cw >>= 1;
-    const uint32_t blocks = (cw * g.m2) / 64 + 1 + 4;
-    const size_t pads = g.m1 == 1 ? 0 : (size_t)blocks * 64 / g.m2;
-    const size_t shm = ((size_t)blocks * 64 + pads + 1) * sizeof(uint32_t);
-    dim3 grid((g.m1 + cw - 1) / cw, ntrials);
-    hipLaunchKernelGGL(k_whiten, grid, dim3(256), shm, s, spec_t, fp16, g.m, g.m1, g.m2, g.nbins,
-                       g.nblocks, g.spec_pitch, partial, npartials, raw_nat, scales, cw);
+                   void* sorted, float* floors, float* scales) {
+    // columns per sorting workgroup: up to SB_BLOCKS natural blocks
+    uint32_t cw = g.m1;
+    while (cw > 1 && (uint64_t)cw * g.m2 > (uint64_t)SB_BLOCKS * 64) cw >>= 1;
+    const uint32_t nbk = (cw * g.m2 + 63) / 64;
+    const size_t shm = (size_t)nbk * 64 * sizeof(uint32_t);
+    const dim3 gs(g.m1 / cw, ntrials), gm((g.nblocks + MB_BLOCKS - 1) / MB_BLOCKS, ntrials);
+    if (fp16) {
+        hipLaunchKernelGGL(k_sort_blocks<true>, gs, dim3(256), shm, s, spec_t, g.m, g.m1, g.m2, cw,
+                           g.nbins, g.spec_pitch, partial, npartials, raw_nat, sorted, floors);
+        hipLaunchKernelGGL(k_block_median<true>, gm, dim3(MB_BLOCKS), 0, s, sorted, spec_t, g.m,
+                           g.nbins, g.nblocks, g.spec_pitch, floors, scales);
+    } else {
+        hipLaunchKernelGGL(k_sort_blocks<false>, gs, dim3(256), shm, s, spec_t, g.m, g.m1, g.m2, cw,
+                           g.nbins, g.spec_pitch, partial, npartials, raw_nat, sorted, floors);
+        hipLaunchKernelGGL(k_block_median<false>, gm, dim3(MB_BLOCKS), 0, s, sorted, spec_t, g.m,
+                           g.nbins, g.nblocks, g.spec_pitch, floors, scales);
+    }
 }
 
 // ----------------------------------------------------- harmonic summing
@@ -231,12 +364,13 @@ struct HarmArgs {
     uint32_t birdie[2 * MAX_BIRDIE_INTERVALS];  // inclusive bin intervals
 };
 
-__device__ __forceinline__ float whitened(const void* raw, int fp16, size_t row, const float* sc,
-                                          uint32_t i, const HarmArgs& a) {
+// Whitened power from a raw value and its block scale: 0 at DC and in birdie
+// bins, else raw / scale (f32 division).
+__device__ __forceinline__ float whiten_val(float r, float scale, uint32_t i, const HarmArgs& a) {
     if (i == 0) return 0.0f;
     for (uint32_t b = 0; b < a.p.nbirdie; b++)
         if (i >= a.birdie[2 * b] && i <= a.birdie[2 * b + 1]) return 0.0f;
-    return raw_at(raw, fp16, row + i) / sc[i >> 6];
+    return r / scale;
 }
 
 // offset of the parabolic peak through raw[k-1..k+1] (lib.rs:87-94), in f64
@@ -252,7 +386,14 @@ __device__ __forceinline__ double interp_offset(double a, double b, double c) {
 
 constexpr int HS_T = 256;       // threads: scores for HS_T bins
 constexpr int HS_OUT = HS_T - 2;  // peak tests per workgroup
+constexpr int HS_NJ = 15;         // harmonics j = 2 .. 16
 
+// One workgroup scores HS_T consecutive bins.  Level 0 (j = 1) is computed for
+// every bin; higher levels only for bins whose whitened power reaches 4
+// (lib.rs:97-99), typically a few percent.  Those bins are compacted into a
+// list, their (bin, harmonic) loads are spread over all threads (one round of
+// memory latency instead of fifteen), and one thread per listed bin then adds
+// its harmonics in order j = 2, 3, ... in f64, as the reference does.
 __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, int fp16,
                                                  uint64_t spec_pitch, uint32_t nbins,
                                                  uint32_t nblocks,
@@ -260,6 +401,11 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
                                                  RawPeak* __restrict__ peaks,
                                                  uint32_t* __restrict__ peak_counts) {
     __shared__ double sc[5][HS_T];
+    __shared__ double lpos[HS_T];   // k + offset of listed bins
+    __shared__ double lsum[HS_T];   // their j = 1 term
+    __shared__ uint32_t lbin[HS_T];  // their thread index
+    __shared__ float hv[HS_T][HS_NJ + 2];  // odd row stride: conflict-free column reads
+    __shared__ uint32_t nlist;
     const uint32_t trial = blockIdx.y;
     const size_t row = (size_t)trial * spec_pitch;
     const float* scl = scales + (size_t)trial * nblocks;
@@ -269,28 +415,53 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
     double offset = 0.0;
 #pragma unroll
     for (int l = 0; l < 5; l++) sc[l][tid] = NEG_INF;
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    bool listed = false;
+    double S1 = 0.0;
     if (k >= (int64_t)a.p.first && k <= (int64_t)a.p.end_h[0]) {
         const uint32_t ku = (uint32_t)k;
-        offset = interp_offset((double)raw_at(raw, fp16, row + ku - 1),
-                               (double)raw_at(raw, fp16, row + ku),
-                               (double)raw_at(raw, fp16, row + ku + 1));
-        const float pwk = whitened(raw, fp16, row, scl, ku, a);
-        double S = 0.0;
-        uint32_t j = 1;
-        for (uint32_t l = 0; l < a.p.nlevels; l++) {
+        const float r0 = raw_at(raw, fp16, row + ku - 1), r1 = raw_at(raw, fp16, row + ku),
+                    r2 = raw_at(raw, fp16, row + ku + 1);
+        const float s0 = scl[(ku - 1) >> 6], s1 = scl[ku >> 6], s2 = scl[(ku + 1) >> 6];
+        offset = interp_offset((double)r0, (double)r1, (double)r2);
+        const float pwk = whiten_val(r1, s1, ku, a);
+        // j = 1 sums the bin at round(k + offset), one of k - 1, k, k + 1
+        uint64_t i1 = (uint64_t)round((double)ku + offset);
+        if (i1 > nbins - 1) i1 = nbins - 1;
+        const float w1 = i1 < ku ? whiten_val(r0, s0, ku - 1, a)
+                                 : (i1 == ku ? pwk : whiten_val(r2, s2, ku + 1, a));
+        S1 = (double)w1;
+        sc[0][tid] = (S1 - 1.0) / a.sqrt_h[0];
+        listed = a.p.nlevels > 1 && k <= (int64_t)a.p.end_h[1] && pwk >= 4.0f;
+    }
+    if (listed) {
+        const uint32_t slot = atomicAdd(&nlist, 1u);
+        lpos[slot] = (double)k + offset;
+        lsum[slot] = S1;
+        lbin[slot] = (uint32_t)tid;
+    }
+    __syncthreads();
+    const uint32_t G = nlist;
+    for (uint32_t it = tid; it < G * HS_NJ; it += HS_T) {
+        const uint32_t g = it / HS_NJ, jj = it - g * HS_NJ;
+        const double position = lpos[g] * (double)(jj + 2);
+        uint64_t index = (uint64_t)round(position);
+        if (index > nbins - 1) index = nbins - 1;
+        const uint32_t iu = (uint32_t)index;
+        hv[g][jj] = whiten_val(raw_at(raw, fp16, row + iu), scl[iu >> 6], iu, a);
+    }
+    __syncthreads();
+    if ((uint32_t)tid < G) {
+        const uint32_t t = lbin[tid];
+        const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)blockIdx.x * HS_OUT + t;
+        double S = lsum[tid];
+        uint32_t j = 2;
+        for (uint32_t l = 1; l < a.p.nlevels; l++) {
             const uint32_t h = 1u << l;
-            if (k > (int64_t)a.p.end_h[l]) break;
-            if (h > 1 && pwk < 4.0f) {
-                sc[l][tid] = NEG_INF;
-                continue;
-            }
-            for (; j <= h; j++) {
-                const double position = ((double)ku + offset) * (double)j;
-                uint64_t index = (uint64_t)round(position);
-                if (index > nbins - 1) index = nbins - 1;
-                S += (double)whitened(raw, fp16, row, scl, (uint32_t)index, a);
-            }
-            sc[l][tid] = (S - (double)h) / a.sqrt_h[l];
+            if (kk > (int64_t)a.p.end_h[l]) break;
+            for (; j <= h; j++) S += (double)hv[tid][j - 2];
+            sc[l][t] = (S - (double)h) / a.sqrt_h[l];
         }
     }
     __syncthreads();
