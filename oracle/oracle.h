@@ -91,6 +91,65 @@ double oracle_refine(const float *series, size_t n, double tsamp, double frequen
                      size_t steps, size_t bins, double *snr, double *chi2, float *values,
                      uint64_t *counts);
 
+/* ---- confirmation stage (crates/search/src/confirmation.rs; oracle/analyze.c) ---- */
+typedef struct {
+    uint64_t start_bin, width_bins, bins;
+} oracle_window;
+typedef struct {
+    double snr;
+    uint64_t on_samples, off_samples;
+    double mean_difference, standard_error;
+} oracle_measurement;
+typedef struct {
+    double minimum_validation_snr, minimum_part_snr;
+    uint64_t minimum_time_parts, minimum_subbands;
+    double minimum_dm_contrast, minimum_dispersion_widths, maximum_zero_dm_spectral_ratio;
+} oracle_rules;
+typedef struct {
+    double dm, spectral_score, spectral_frequency_hz, discovery_profile_response;
+    uint32_t has_spectral, pad_;
+} oracle_dm_point;
+typedef struct {
+    uint32_t family, pad_;
+    uint64_t index; /* into the generated candidates */
+} oracle_member;
+typedef struct {
+    oracle_candidate candidate;
+    double refined_dm;
+    double profile_period_s, profile_snr, profile_chi2;
+    oracle_window window;
+    oracle_measurement validation, zero_dm_validation;
+    double zero_dm_period_s;
+    oracle_window zero_dm_window;
+    double dm_contrast, dispersion_widths, zero_dm_related_spectral_ratio;
+    oracle_measurement time_parts[4], subbands[4];
+    uint64_t nmembers;
+    uint32_t accepted, rejection_mask; /* bit i = i-th reason of confirmation.rs:434-466 */
+} oracle_confirmed;
+typedef struct {
+    uint64_t discovery_samples, validation_start_sample, validation_samples, common_samples,
+        trial_count, ncandidates;
+} oracle_outcome;
+
+/* confirmation.rs:107-141 */
+void oracle_measure(const float *x, size_t n, size_t offset, double tsamp, double period,
+                    const oracle_window *w, oracle_measurement *out);
+/* confirmation.rs:142-176 */
+void oracle_select_window(const float *values, size_t bins, oracle_window *out);
+/* confirmation.rs:177-199.  Returns the family count. */
+size_t oracle_group(const oracle_candidate *in, size_t n, double resolution, size_t limit,
+                    oracle_candidate *leaders, oracle_member *members, size_t *nmembers);
+/* confirmation.rs:203-496 over f32 channel-major data.  Arrays are sized by
+ * limit (conf), limit*bins (profile_*), limit*4*bins (time/subband profiles),
+ * limit*ntrials (dm_response) and ngen (members).  Returns 0 or -1 (err set). */
+int oracle_analyze(const float *data, size_t nsamples, const double *freqs, size_t nch, double tsamp,
+                   const double *trials, size_t ntrials, const oracle_candidate *generated,
+                   size_t ngen, size_t limit, size_t bins, const oracle_rules *rules,
+                   oracle_outcome *outcome, oracle_confirmed *conf, float *profile_bins,
+                   uint64_t *profile_counts, float *time_profiles, float *subband_profiles,
+                   oracle_dm_point *dm_response, oracle_member *members, size_t *nmembers, char *err,
+                   size_t errlen);
+
 /* Whole per-trial loop of crates/cli/src/main.rs:462-469 over f32 channel-major data,
  * on `threads` POSIX threads (the CPU baseline).  counts[ntrials], out[ntrials*cap]. */
 int oracle_search_trials(const float *data, size_t nsamples, const double *freqs, size_t nch,

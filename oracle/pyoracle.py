@@ -229,3 +229,175 @@ def search_trials(data, nsamples, freqs, tsamp, dms, length, threads=1, **cfg):
                          dm=out[t * cap + i].dm, harmonics=int(out[t * cap + i].harmonics), score=out[t * cap + i].score)
                     for i in range(int(counts[t]))])
     return res
+
+
+# ------------------------------------------------ confirmation (analyze.c)
+class OWindow(C.Structure):
+    _fields_ = [("start_bin", C.c_uint64), ("width_bins", C.c_uint64), ("bins", C.c_uint64)]
+
+
+class OMeasurement(C.Structure):
+    _fields_ = [("snr", C.c_double), ("on_samples", C.c_uint64), ("off_samples", C.c_uint64),
+                ("mean_difference", C.c_double), ("standard_error", C.c_double)]
+
+
+class ORules(C.Structure):
+    _fields_ = [("minimum_validation_snr", C.c_double), ("minimum_part_snr", C.c_double),
+                ("minimum_time_parts", C.c_uint64), ("minimum_subbands", C.c_uint64),
+                ("minimum_dm_contrast", C.c_double), ("minimum_dispersion_widths", C.c_double),
+                ("maximum_zero_dm_spectral_ratio", C.c_double)]
+
+
+class ODmPoint(C.Structure):
+    _fields_ = [("dm", C.c_double), ("spectral_score", C.c_double), ("spectral_frequency_hz", C.c_double),
+                ("discovery_profile_response", C.c_double), ("has_spectral", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class OMember(C.Structure):
+    _fields_ = [("family", C.c_uint32), ("pad_", C.c_uint32), ("index", C.c_uint64)]
+
+
+class OConfirmed(C.Structure):
+    _fields_ = [("candidate", OCand), ("refined_dm", C.c_double), ("profile_period_s", C.c_double),
+                ("profile_snr", C.c_double), ("profile_chi2", C.c_double), ("window", OWindow),
+                ("validation", OMeasurement), ("zero_dm_validation", OMeasurement), ("zero_dm_period_s", C.c_double),
+                ("zero_dm_window", OWindow), ("dm_contrast", C.c_double), ("dispersion_widths", C.c_double),
+                ("zero_dm_related_spectral_ratio", C.c_double), ("time_parts", OMeasurement * 4),
+                ("subbands", OMeasurement * 4), ("nmembers", C.c_uint64), ("accepted", C.c_uint32),
+                ("rejection_mask", C.c_uint32)]
+
+
+class OOutcome(C.Structure):
+    _fields_ = [("discovery_samples", C.c_uint64), ("validation_start_sample", C.c_uint64),
+                ("validation_samples", C.c_uint64), ("common_samples", C.c_uint64), ("trial_count", C.c_uint64),
+                ("ncandidates", C.c_uint64)]
+
+
+# crates/search/src/confirmation.rs:434-466, in bit order of rejection_mask
+REJECTION_REASONS = [
+    "held-out pulse SNR below threshold",
+    "pulse does not persist in enough held-out time segments",
+    "pulse does not persist in enough held-out subbands",
+    "insufficient improvement over zero-DM interference control",
+    "dispersion sweep too small relative to selected pulse width to distinguish local interference",
+    "stronger rationally related periodic signal in zero-DM discovery control",
+]
+DEFAULT_RULES = dict(minimum_validation_snr=6.0, minimum_part_snr=1.0, minimum_time_parts=3, minimum_subbands=3,
+                     minimum_dm_contrast=1.25, minimum_dispersion_widths=1.5, maximum_zero_dm_spectral_ratio=1.0)
+
+
+def _bind_confirmation(L):
+    fp, up, dp = C.POINTER(C.c_float), C.POINTER(C.c_uint64), C.POINTER(C.c_double)
+    L.oracle_measure.restype = None
+    L.oracle_measure.argtypes = [fp, C.c_size_t, C.c_size_t, C.c_double, C.c_double, C.POINTER(OWindow),
+                                 C.POINTER(OMeasurement)]
+    L.oracle_select_window.restype = None
+    L.oracle_select_window.argtypes = [fp, C.c_size_t, C.POINTER(OWindow)]
+    L.oracle_group.restype = C.c_size_t
+    L.oracle_group.argtypes = [C.POINTER(OCand), C.c_size_t, C.c_double, C.c_size_t, C.POINTER(OCand),
+                               C.POINTER(OMember), C.POINTER(C.c_size_t)]
+    L.oracle_analyze.restype = C.c_int
+    L.oracle_analyze.argtypes = [fp, C.c_size_t, dp, C.c_size_t, C.c_double, dp, C.c_size_t, C.POINTER(OCand),
+                                 C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(ORules), C.POINTER(OOutcome),
+                                 C.POINTER(OConfirmed), fp, up, fp, fp, C.POINTER(ODmPoint), C.POINTER(OMember),
+                                 C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]
+
+
+def _clib():
+    L = lib()
+    if not getattr(L, "_conf_bound", False):
+        _bind_confirmation(L)
+        L._conf_bound = True
+    return L
+
+
+def _cand_array(cands):
+    arr = (OCand * max(1, len(cands)))()
+    for i, c in enumerate(cands):
+        arr[i] = OCand(c["frequency_hz"], c["period_s"], c["dm"], int(c["harmonics"]), c["score"])
+    return arr
+
+
+def _meas(m):
+    return dict(snr=m.snr, on_samples=int(m.on_samples), off_samples=int(m.off_samples),
+                mean_difference=m.mean_difference, standard_error=m.standard_error)
+
+
+def _win(w):
+    return dict(start_bin=int(w.start_bin), width_bins=int(w.width_bins), bins=int(w.bins))
+
+
+def measure(x, offset, tsamp, period, window):
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    w = OWindow(window["start_bin"], window["width_bins"], window["bins"])
+    m = OMeasurement()
+    _clib().oracle_measure(_f(a), len(a), offset, tsamp, period, C.byref(w), C.byref(m))
+    return _meas(m)
+
+
+def select_window(values):
+    a = np.ascontiguousarray(values, dtype=np.float32)
+    w = OWindow()
+    _clib().oracle_select_window(_f(a), len(a), C.byref(w))
+    return _win(w)
+
+
+def group(cands, resolution, limit):
+    """-> (leaders, members): members lists (family, index into cands) in append order."""
+    arr = _cand_array(cands)
+    leaders = (OCand * max(1, limit))()
+    members = (OMember * max(1, len(cands)))()
+    nm = C.c_size_t()
+    nf = _clib().oracle_group(arr, len(cands), resolution, limit, leaders, members, C.byref(nm))
+    return _cands(leaders, nf), [(int(members[i].family), int(members[i].index)) for i in range(nm.value)]
+
+
+def analyze(data, nsamples, freqs, tsamp, trials, generated, limit, bins, rules=None):
+    """confirmation::analyze over f32 channel-major data -> the Outcome as a dict."""
+    d = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    tr = np.ascontiguousarray(trials, dtype=np.float64)
+    r = dict(DEFAULT_RULES, **(rules or {}))
+    R = ORules(**r)
+    gen = _cand_array(generated)
+    out = OOutcome()
+    conf = (OConfirmed * limit)()
+    pb = np.zeros(limit * bins, np.float32)
+    pc = np.zeros(limit * bins, np.uint64)
+    tp = np.zeros(limit * 4 * bins, np.float32)
+    sp = np.zeros(limit * 4 * bins, np.float32)
+    dmr = (ODmPoint * (limit * len(tr)))()
+    mem = (OMember * max(1, len(generated)))()
+    nm = C.c_size_t()
+    err = C.create_string_buffer(256)
+    rc = _clib().oracle_analyze(_f(d), nsamples, _d(f), len(f), tsamp, _d(tr), len(tr), gen, len(generated), limit,
+                                bins, C.byref(R), C.byref(out), conf, _f(pb), _u(pc), _f(tp), _f(sp), dmr, mem,
+                                C.byref(nm), err, 256)
+    if rc:
+        raise ValueError(err.value.decode())
+    members = [(int(mem[i].family), int(mem[i].index)) for i in range(nm.value)]
+    cands = []
+    for k in range(int(out.ncandidates)):
+        o = conf[k]
+        cands.append(dict(
+            candidate=dict(frequency_hz=o.candidate.frequency_hz, period_s=o.candidate.period_s,
+                           dm=o.candidate.dm, harmonics=int(o.candidate.harmonics), score=o.candidate.score),
+            refined_dm=o.refined_dm,
+            profile=dict(period_s=o.profile_period_s, snr=o.profile_snr, reduced_chi_square=o.profile_chi2,
+                         bins=pb[k * bins:(k + 1) * bins].copy(), counts=pc[k * bins:(k + 1) * bins].copy()),
+            window=_win(o.window), validation=_meas(o.validation), zero_dm_validation=_meas(o.zero_dm_validation),
+            zero_dm_period_s=o.zero_dm_period_s, zero_dm_window=_win(o.zero_dm_window), dm_contrast=o.dm_contrast,
+            dispersion_widths=o.dispersion_widths, zero_dm_related_spectral_ratio=o.zero_dm_related_spectral_ratio,
+            time_parts=[_meas(o.time_parts[i]) for i in range(4)], subbands=[_meas(o.subbands[i]) for i in range(4)],
+            time_profiles=[tp[(k * 4 + i) * bins:(k * 4 + i + 1) * bins].copy() for i in range(4)],
+            subband_profiles=[sp[(k * 4 + i) * bins:(k * 4 + i + 1) * bins].copy() for i in range(4)],
+            dm_response=[dict(dm=p.dm, spectral_score=p.spectral_score if p.has_spectral else None,
+                              spectral_frequency_hz=p.spectral_frequency_hz if p.has_spectral else None,
+                              discovery_profile_response=p.discovery_profile_response)
+                         for p in dmr[k * len(tr):(k + 1) * len(tr)]],
+            harmonic_members=[generated[i] for fam, i in members if fam == k],
+            accepted=bool(o.accepted),
+            rejection_reasons=[REJECTION_REASONS[b] for b in range(6) if o.rejection_mask >> b & 1]))
+    return dict(discovery_samples=int(out.discovery_samples), validation_start_sample=int(out.validation_start_sample),
+                validation_samples=int(out.validation_samples), common_samples=int(out.common_samples),
+                trial_count=int(out.trial_count), candidates=cands)

@@ -61,6 +61,29 @@ def members(report):
                        refined_dm=cc["refined_dm"]) for cc in c["candidates"]])
 
 
+def confirmation(report, families=2, dm_every=8):
+    """The first `families` confirmed candidates of a report, for pinning analyze.
+
+    dm_response keeps every `dm_every`-th trial (and the last) to stay small."""
+    r = json.load(open(report))
+    c = r["confirmation"]
+    keep = ("candidate", "refined_dm", "profile", "window", "validation", "zero_dm_validation", "zero_dm_period_s",
+            "zero_dm_window", "dm_contrast", "dispersion_widths", "zero_dm_related_spectral_ratio", "time_parts",
+            "subbands", "time_profiles", "subband_profiles", "accepted", "rejection_reasons")
+    out = []
+    for cc in c["candidates"][:families]:
+        d = {k: cc[k] for k in keep}
+        n = len(cc["dm_response"])
+        idx = sorted(set(list(range(0, n, dm_every)) + [n - 1]))
+        d["dm_response_index"] = idx
+        d["dm_response"] = [cc["dm_response"][i] for i in idx]
+        d["nmembers"] = len(cc["harmonic_members"])
+        out.append(d)
+    head = {k: c[k] for k in ("discovery_samples", "validation_start_sample", "validation_samples",
+                              "common_samples", "trial_count")}
+    return dict(outcome=head, rules=c["rules"], families=out)
+
+
 def main():
     gbt = header(os.path.join(REF, "GBT_Lband_PSR.fil"))
     syn = header(os.path.join(REF, "synthetic.fil"))
@@ -77,6 +100,10 @@ def main():
               open(os.path.join(OUT, "synthetic_report.json"), "w"), separators=(",", ":"))
     json.dump(members(os.path.join(REF, "output_confirm_gbt/report.json")),
               open(os.path.join(OUT, "gbt_report.json"), "w"), separators=(",", ":"))
+    json.dump(confirmation(os.path.join(REF, "output_confirm_gbt/report.json")),
+              open(os.path.join(OUT, "gbt_confirm.json"), "w"), separators=(",", ":"))
+    json.dump(confirmation(os.path.join(REF, "output_confirm_synth/report.json")),
+              open(os.path.join(OUT, "synthetic_confirm.json"), "w"), separators=(",", ":"))
 
 
 if __name__ == "__main__":
