@@ -180,6 +180,74 @@ int psr_refine(psr_ctx *ctx, const float *series, size_t n, double tsamp, double
                double span_hz, size_t steps, uint32_t bins, psr_profile_stats *stats,
                float *values, uint64_t *counts);
 
+/* ---- confirmation (crates/search/src/confirmation.rs) ------------------- */
+/* Window (confirmation.rs:41-51) */
+typedef struct {
+    uint64_t start_bin, width_bins, bins;
+} psr_window;
+/* Measurement (confirmation.rs:52-59) */
+typedef struct {
+    double snr;
+    uint64_t on_samples, off_samples;
+    double mean_difference, standard_error;
+} psr_measurement;
+/* Rules (confirmation.rs:17-40); psr_rules_default fills Rules::default(). */
+typedef struct {
+    double minimum_validation_snr, minimum_part_snr;
+    uint64_t minimum_time_parts, minimum_subbands;
+    double minimum_dm_contrast, minimum_dispersion_widths, maximum_zero_dm_spectral_ratio;
+} psr_rules;
+/* DmPoint (confirmation.rs:60-69); has_spectral = 0 stands for None. */
+typedef struct {
+    double dm, spectral_score, spectral_frequency_hz, discovery_profile_response;
+    uint32_t has_spectral, pad_;
+} psr_dm_point;
+/* One harmonic member: its family and its index in the input candidates. */
+typedef struct {
+    uint32_t family, pad_;
+    uint64_t index;
+} psr_member;
+/* ConfirmedCandidate (confirmation.rs:70-90) without its variable-length parts,
+ * which psr_analyze writes to caller arrays.  Bit i of rejection_mask is the
+ * i-th rejection reason of confirmation.rs:434-466, in source order. */
+typedef struct {
+    psr_candidate candidate;
+    double refined_dm;
+    psr_profile_stats profile;
+    psr_window window;
+    psr_measurement validation, zero_dm_validation;
+    double zero_dm_period_s;
+    psr_window zero_dm_window;
+    double dm_contrast, dispersion_widths, zero_dm_related_spectral_ratio;
+    psr_measurement time_parts[4], subbands[4];
+    uint64_t nmembers;
+    uint32_t accepted, rejection_mask;
+} psr_confirmed;
+/* Outcome (confirmation.rs:91-100) */
+typedef struct {
+    uint64_t discovery_samples, validation_start_sample, validation_samples, common_samples,
+        trial_count, ncandidates;
+} psr_outcome;
+
+void psr_rules_default(psr_rules *out);
+/* confirmation::group (confirmation.rs:177-199): families of harmonically
+ * related candidates, strongest first.  leaders[limit]; members[n] lists
+ * (family, input index) in the order members were appended. */
+int psr_group(const psr_candidate *in, size_t n, double resolution_hz, size_t limit,
+              psr_candidate *leaders, size_t *nfamilies, psr_member *members, size_t *nmembers);
+/* confirmation::analyze (confirmation.rs:203-496) over a device-resident
+ * observation: every dedispersion, fold, channel profile and DM response runs
+ * on the GPU; window selection, measurements and rules on the host.  Caller
+ * arrays: confirmed[limit], profile_bins/profile_counts[limit*bins],
+ * time_profiles/subband_profiles[limit*4*bins], dm_response[limit*ntrials],
+ * members[ngenerated]. */
+int psr_analyze(psr_ctx *ctx, psr_obs *obs, const double *trials, size_t ntrials,
+                const psr_candidate *generated, size_t ngenerated, size_t limit, uint32_t bins,
+                const psr_rules *rules, psr_outcome *outcome, psr_confirmed *confirmed,
+                float *profile_bins, uint64_t *profile_counts, float *time_profiles,
+                float *subband_profiles, psr_dm_point *dm_response, psr_member *members,
+                size_t *nmembers);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,46 @@ class ProfileStats(C.Structure):
     _fields_ = [("period_s", C.c_double), ("snr", C.c_double), ("reduced_chi_square", C.c_double)]
 
 
+class Window(C.Structure):  # psr_window
+    _fields_ = [("start_bin", C.c_uint64), ("width_bins", C.c_uint64), ("bins", C.c_uint64)]
+
+
+class Measurement(C.Structure):  # psr_measurement
+    _fields_ = [("snr", C.c_double), ("on_samples", C.c_uint64), ("off_samples", C.c_uint64),
+                ("mean_difference", C.c_double), ("standard_error", C.c_double)]
+
+
+class Rules(C.Structure):  # psr_rules
+    _fields_ = [("minimum_validation_snr", C.c_double), ("minimum_part_snr", C.c_double),
+                ("minimum_time_parts", C.c_uint64), ("minimum_subbands", C.c_uint64),
+                ("minimum_dm_contrast", C.c_double), ("minimum_dispersion_widths", C.c_double),
+                ("maximum_zero_dm_spectral_ratio", C.c_double)]
+
+
+class DmPoint(C.Structure):  # psr_dm_point
+    _fields_ = [("dm", C.c_double), ("spectral_score", C.c_double), ("spectral_frequency_hz", C.c_double),
+                ("discovery_profile_response", C.c_double), ("has_spectral", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class Member(C.Structure):  # psr_member
+    _fields_ = [("family", C.c_uint32), ("pad_", C.c_uint32), ("index", C.c_uint64)]
+
+
+class Confirmed(C.Structure):  # psr_confirmed
+    _fields_ = [("candidate", Candidate), ("refined_dm", C.c_double), ("profile", ProfileStats),
+                ("window", Window), ("validation", Measurement), ("zero_dm_validation", Measurement),
+                ("zero_dm_period_s", C.c_double), ("zero_dm_window", Window), ("dm_contrast", C.c_double),
+                ("dispersion_widths", C.c_double), ("zero_dm_related_spectral_ratio", C.c_double),
+                ("time_parts", Measurement * 4), ("subbands", Measurement * 4), ("nmembers", C.c_uint64),
+                ("accepted", C.c_uint32), ("rejection_mask", C.c_uint32)]
+
+
+class Outcome(C.Structure):  # psr_outcome
+    _fields_ = [("discovery_samples", C.c_uint64), ("validation_start_sample", C.c_uint64),
+                ("validation_samples", C.c_uint64), ("common_samples", C.c_uint64), ("trial_count", C.c_uint64),
+                ("ncandidates", C.c_uint64)]
+
+
 class StageTimes(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("delays_ms", C.c_double), ("dedisperse_ms", C.c_double),
                 ("fft_ms", C.c_double), ("whiten_ms", C.c_double), ("harmsum_ms", C.c_double),
@@ -97,6 +137,14 @@ EXPORTS = [
     ("psr_spectrum", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(SearchCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
     ("psr_fold", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     ("psr_refine", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
+    ("psr_rules_default", None, [C.POINTER(Rules)]),
+    ("psr_group", C.c_int, [C.POINTER(Candidate), C.c_size_t, C.c_double, C.c_size_t, C.POINTER(Candidate),
+                            C.POINTER(C.c_size_t), C.POINTER(Member), C.POINTER(C.c_size_t)]),
+    ("psr_analyze", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.POINTER(Candidate),
+                              C.c_size_t, C.c_size_t, C.c_uint32, C.POINTER(Rules), C.POINTER(Outcome),
+                              C.POINTER(Confirmed), C.POINTER(C.c_float), C.POINTER(C.c_uint64),
+                              C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(DmPoint), C.POINTER(Member),
+                              C.POINTER(C.c_size_t)]),
 ]
 
 _lib = None

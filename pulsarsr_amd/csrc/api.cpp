@@ -11,58 +11,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/psr_gpu.h"
-#include "common.hpp"
-#include "host_internal.hpp"
+#include "ctx_internal.hpp"
 
 using namespace psr;
 
-namespace {
-
-#define HIP_TRY(expr)                                                                    \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return fail(e_ == hipErrorOutOfMemory ? PSR_E_OOM : PSR_E_DEVICE,            \
-                        std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-}  // namespace
-
-struct psr_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint64_t mem_cap = 0;
-    uint64_t mem_used = 0;
-    DevBuf delays, dd_table, dms, series, dd_partial, mean32, corr16, work, spec_t, raw_nat, scales,
-        blk_sorted, floors,
-        fft_partial, peaks, peak_counts, cands, counts, twiddles, costabs, birdies, fold_series, fold_per,
-        fold_off, fold_sums, fold_counts;
-    SpecGeom tw_geom{};
-    bool tw_valid = false;
-    hipEvent_t ev[8] = {};
-    psr_stage_times last{};
-};
-
-struct psr_obs {
-    psr_ctx* ctx = nullptr;
-    int kind = KIND_U8;
-    uint32_t nchans = 0, nbits = 0, maxval = 0;
-    uint64_t nsamples = 0, pitch = 0;
-    double tsamp = 0, fch1 = 0, foff = 0;
-    void* data = nullptr;
-    double* inv_f2 = nullptr;  // device 1/(f*f) per channel
-    std::vector<double> freqs;
-    double ref_inv = 0;
-    double m0 = 0, s0 = 1;  // FP16 storage offset and scale
-};
-
-namespace {
+namespace psr {
 
 int ensure(psr_ctx* c, DevBuf& b, size_t bytes) {
     if (b.bytes >= bytes) return PSR_OK;
@@ -97,6 +50,10 @@ int check_device(psr_ctx* c) {
     HIP_TRY(hipSetDevice(c->device));
     return PSR_OK;
 }
+
+}  // namespace psr
+
+namespace {
 
 // apply()'s length rules (crates/dedisperse/src/lib.rs:72-78) for the largest DM
 // of the batch; delays are non-decreasing in DM so it bounds every trial.
@@ -331,7 +288,8 @@ void psr_ctx_destroy(psr_ctx* c) {
                      &c->corr16, &c->work, &c->spec_t, &c->raw_nat, &c->scales, &c->blk_sorted,
                      &c->floors, &c->fft_partial, &c->peaks, &c->peak_counts, &c->cands,
                      &c->counts, &c->twiddles, &c->costabs, &c->birdies,
-                     &c->fold_series, &c->fold_per, &c->fold_off, &c->fold_sums, &c->fold_counts};
+                     &c->fold_series, &c->fold_per, &c->fold_off, &c->fold_sums, &c->fold_counts,
+                     &c->an_values, &c->an_means, &c->an_resp};
     for (DevBuf* b : all) release(c, *b);
     for (int i = 0; i < 8; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -619,6 +577,14 @@ int psr_dedisperse(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, si
     if ((rc = resolve_length(o, dms, ntrials, length, &n, &maxd))) return rc;
     *out_len = n;
     if (!out) return PSR_OK;
+    return dedisperse_to_host(c, o, dms, ntrials, n, out);
+}
+
+}  // extern "C"
+
+int psr::dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t n,
+                            float* out) {
+    int rc;
     std::vector<size_t> order(ntrials);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return dms[a] < dms[b]; });
@@ -641,6 +607,8 @@ int psr_dedisperse(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, si
     }
     return PSR_OK;
 }
+
+extern "C" {
 
 // ------------------------------------------------------------------ search
 int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t length,

@@ -113,6 +113,14 @@ void launch_trial_mean(hipStream_t s, const double* partial, uint32_t ntiles, ui
 void launch_fold(hipStream_t s, const float* series, uint64_t n, double tsamp,
                  const double* periods, const uint64_t* offsets, uint32_t nprof, uint32_t bins,
                  double* sums, uint64_t* counts);
+// confirmation stage: per-channel profiles of raw rows and the DM response
+void launch_fold_rows(hipStream_t s, const void* data, int kind, uint64_t pitch, uint32_t nrows,
+                      uint64_t n, double tsamp, double period, uint32_t bins, double* sums,
+                      uint64_t* counts);
+void launch_dm_response(hipStream_t s, const float* values, const double* means,
+                        const int32_t* delays, uint32_t nch, uint32_t ntrials, uint32_t bins,
+                        double tsamp, double period, uint32_t start, uint32_t width,
+                        double* resp);
 
 // Twiddle tables needed by launch_fft for a geometry (size in float2 elements).
 size_t twiddle_table_size(const SpecGeom& g);

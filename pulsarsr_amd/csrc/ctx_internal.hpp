@@ -1,0 +1,69 @@
+// Internal view of the C ABI's context and observation objects, shared by the
+// translation units that implement include/psr_gpu.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/psr_gpu.h"
+#include "common.hpp"
+#include "host_internal.hpp"
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return psr::fail(e_ == hipErrorOutOfMemory ? PSR_E_OOM : PSR_E_DEVICE,       \
+                             std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace psr {
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+}  // namespace psr
+
+struct psr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t mem_cap = 0;
+    uint64_t mem_used = 0;
+    psr::DevBuf delays, dd_table, dms, series, dd_partial, mean32, corr16, work, spec_t, raw_nat,
+        scales, blk_sorted, floors, fft_partial, peaks, peak_counts, cands, counts, twiddles, costabs,
+        birdies, fold_series, fold_per, fold_off, fold_sums, fold_counts, an_values, an_means,
+        an_resp;
+    psr::SpecGeom tw_geom{};
+    bool tw_valid = false;
+    hipEvent_t ev[8] = {};
+    psr_stage_times last{};
+};
+
+struct psr_obs {
+    psr_ctx* ctx = nullptr;
+    int kind = psr::KIND_U8;
+    uint32_t nchans = 0, nbits = 0, maxval = 0;
+    uint64_t nsamples = 0, pitch = 0;
+    double tsamp = 0, fch1 = 0, foff = 0;
+    void* data = nullptr;
+    double* inv_f2 = nullptr;  // device 1/(f*f) per channel
+    std::vector<double> freqs;
+    double ref_inv = 0;
+    double m0 = 0, s0 = 1;  // FP16 storage offset and scale
+};
+
+namespace psr {
+// device buffer growth within the ctx's memory cap
+int ensure(psr_ctx* c, DevBuf& b, size_t bytes);
+void release(psr_ctx* c, DevBuf& b);
+int check_device(psr_ctx* c);
+// FP32 dedispersion of `ntrials` DMs (any order) at output length n into host
+// rows out[i * n ..]; the caller has validated n against the delays.
+int dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t n,
+                       float* out);
+// bytes per sample of an observation's channel rows
+inline size_t sample_bytes(const psr_obs* o) {
+    return o->kind == KIND_U8 ? 1 : (o->kind == KIND_U16 ? 2 : 4);
+}
+}  // namespace psr

@@ -113,3 +113,21 @@ def test_sift_reference_case():
     c = search.sift([mk(100.0, 20.0, 10.0), mk(200.0, 21.0, 8.0), mk(100.01, 19.0, 9.0), mk(137.0, 0.0, 20.0)],
                     0.1, 10, True)
     assert len(c) == 1 and c[0].dm == 20.0
+
+
+def test_group_matches_oracle_and_rules_default():
+    # confirmation::group (crates/search/src/confirmation.rs:177-199) is host code
+    from pulsarsr_amd import confirmation
+    rng = np.random.default_rng(7)
+    base = rng.uniform(1.0, 50.0, 12)
+    cands = []
+    for i in range(400):
+        f = base[i % 12] * rng.integers(1, 5) / rng.integers(1, 3) + rng.normal(0, 1e-3)
+        s = float(np.float32(rng.uniform(6, 40)))  # f32-valued scores give exact ties
+        cands.append(dict(frequency_hz=f, period_s=1.0 / f, dm=float(rng.integers(0, 40)) * 0.5, harmonics=1,
+                          score=s if i % 7 else 12.0))
+    for limit in (1, 5, 50):
+        got = confirmation.group(cands, 0.01, limit)
+        want = oracle.group(cands, 0.01, limit)
+        assert got == want
+    assert confirmation.default_rules() == oracle.DEFAULT_RULES
