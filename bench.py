@@ -23,7 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TADDS = 256 * 64 * 2.4e9 / 1e12  # 32-bit adds: 4 SIMD x 16 lanes per CU per clock
+# MI355X: 256 CUs at 2.4 GHz.  VALU: 64 lanes per CU per clock; v_pk_add_u16
+# (the dedispersion add) does two 16-bit adds per lane.  LDS: 128 B per CU per clock.
+VALU_PEAK_TADDS = 256 * 64 * 2 * 2.4e9 / 1e12
+LDS_PEAK_GBS = 256 * 128 * 2.4e9 / 1e9
 
 
 def log(*a):
@@ -46,14 +49,60 @@ def run_search(obs, w, n, precision):
     return dt, out, counts, cap, obs.ctx.stage_times()
 
 
-def top_list(out, counts, cap, k=20):
+def all_candidates(out, counts, cap):
+    """Generated candidates in trial order (the `generated` list of confirmation::analyze)."""
     c = []
     for t in range(len(counts)):
         for i in range(int(counts[t])):
             x = out[t * cap + i]
-            c.append((x.score, x.frequency_hz, x.dm, int(x.harmonics)))
-    c.sort(key=lambda v: -v[0])
-    return c[:k]
+            c.append(dict(frequency_hz=x.frequency_hz, period_s=x.period_s, dm=x.dm, harmonics=int(x.harmonics),
+                          score=x.score))
+    return c
+
+
+def top_families(cands, n, tsamp, k=20):
+    """Leaders of the k strongest harmonic families (confirmation.rs:177-199 with
+    analyze's resolution 1.5 / (N_d tsamp)): the report's candidate list."""
+    from pulsarsr_amd import confirmation
+    leaders, _ = confirmation.group(cands, 1.5 / (n * tsamp), k)
+    return leaders
+
+
+# HBM bytes per unit of each kernel on the FP16 path (FETCH_SIZE + WRITE_SIZE,
+# calibrated against a 1 GiB device copy), from the rocprofv3 passes recorded in
+# profiles/hbm_traffic_headline-1024ch-8bit.md (960-trial launches).
+MEASURED_TRAFFIC_BYTES_PER_UNIT = {
+    "k_dedisp_u8": 14.54,
+    "k_fft_pass1": 12.67,
+    "k_fft_pass2": 11.51,
+    "k_sort_blocks": 8.08,
+    "k_block_median": 2.15,
+    "k_harmsum": 4.22,
+}
+
+
+def traffic_per_unit():
+    return MEASURED_TRAFFIC_BYTES_PER_UNIT
+
+
+def fp64_error(obs, w, n, out16, c16, cap, ntrials):
+    """Candidate-SNR error of the FP16 search against the FP64 oracle on the
+    trials holding the strongest FP16 candidates (full-length series)."""
+    import oracle
+    from pulsarsr_amd import dedisperse
+    best = sorted(range(len(c16)), key=lambda t: -max([out16[t * cap + i].score for i in range(int(c16[t]))] or [0]))
+    worst, matched = 0.0, 0
+    for t in best[:ntrials]:
+        x = dedisperse.apply_dms(obs, [float(w.dms[t])], n)[0]
+        ref = oracle.periodicity(x, w.tsamp, float(w.dms[t]), f64=True, min_hz=w.min_hz, max_hz=w.max_hz,
+                                 threshold=w.threshold, max_candidates=32, max_harmonics=w.max_harmonics)
+        for i in range(int(c16[t])):
+            g = out16[t * cap + i]
+            hit = [r for r in ref if r["harmonics"] == g.harmonics and abs(r["frequency_hz"] - g.frequency_hz) < 1e-6 * g.frequency_hz]
+            if hit:
+                worst = max(worst, abs(g.score - hit[0]["score"]) / abs(hit[0]["score"]))
+                matched += 1
+    return dict(max_rel_snr_error=worst, matched=matched, trials=ntrials)
 
 
 def cpu_baseline(payload, w, n, ntrials, threads):
@@ -81,6 +130,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--trials", type=int, default=0, help="use the first N DM trials only (profiling runs)")
+    ap.add_argument("--fp64-trials", type=int, default=2, help="trials checked against the FP64 oracle")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (HBM counter calibration for rocprofv3)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
@@ -88,6 +140,14 @@ def main():
         sys.exit(2)
 
     from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, synth
+    if a.copy_calibration_mib:
+        import torch
+        src = torch.ones(a.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB" % a.copy_calibration_mib)
+        del src, dst
     w = synth.workload(a.workload)
     if a.trials:
         w.dms = w.dms[:a.trials]
@@ -116,16 +176,29 @@ def main():
         t32.append(dt)
         log("fp32 step %d: %.1f ms" % (i, dt * 1e3))
     ms32 = 1e3 * float(np.median(t32))
-    top16, top32 = top_list(out16, c16, cap), top_list(out32, c32, cap32)
-    same_top = [(round(x[1], 6), x[2], x[3]) for x in top16] == [(round(x[1], 6), x[2], x[3]) for x in top32]
+    fam16 = top_families(all_candidates(out16, c16, cap), n, w.tsamp)
+    fam32 = top_families(all_candidates(out32, c32, cap32), n, w.tsamp)
+    key = lambda f: [(round(x["frequency_hz"], 6), x["dm"], x["harmonics"]) for x in f]
+    same_top = key(fam16) == key(fam32)
+    snr_err = max([abs(x["score"] - y["score"]) / abs(y["score"]) for x, y in zip(fam16, fam32)] or [0.0])
 
-    # dominant kernel: dedispersion, VALU-bound (1 add per channel per output)
+    # dominant kernel: dedispersion (one packed 16-bit add per channel and output;
+    # the add's operand comes from LDS, 2 bytes per add)
     dd_ms = stages["dedisperse_ms"]
     adds = float(units) * w.nchans
     ach_tadds = adds / (dd_ms * 1e-3) / 1e12
-    # spectral chain bytes (FP16 storage): series write+read 2+2, spectrum 2r+2r per unit
+    ach_lds = adds * 2 / (dd_ms * 1e-3) / 1e9
+    tpu = traffic_per_unit()
+    launches = int(stages.get("launches", 0)) or 1
+    units_per_launch = float(units) / launches
+    dd_traffic = tpu.get("k_dedisp_u8")
+    # spectral chain bytes (FP16 storage): series read 2, spectrum write 2r, whitening
+    # read 2r + write 2r, harmonic-sum read 2r per unit (r = bins per series sample)
     r = (stages["fft_len"] // 2 + 1) / n
     spec_ms = stages["fft_ms"] + stages["whiten_ms"] + stages["harmsum_ms"]
+    spec_alg = 2 + 2 * r + 2 * r + 2 * r
+    spec_traffic = sum(tpu.get(k, 0.0) for k in ("k_fft_pass1", "k_fft_pass2", "k_sort_blocks", "k_block_median",
+                                                  "k_harmsum")) or None
     res = {
         "metric": "DM-trial·samples/s (dedisp+FFT+harmsum), FP16-store vs FP32 at same candidates",
         "value": units / (ms16 * 1e-3),
@@ -138,17 +211,24 @@ def main():
                    "harmonics": w.max_harmonics, "storage": "fp16"},
         "fp32": {"value": units / (ms32 * 1e-3), "ms_per_step": ms32},
         "fp16_over_fp32": ms32 / ms16,
-        "top20_identical_fp16_vs_fp32": same_top,
+        "top20_families_identical_fp16_vs_fp32": same_top,
+        "top20_max_rel_snr_diff_fp16_vs_fp32": snr_err,
         "stages_ms": {k: stages[k] for k in ("delays_ms", "dedisperse_ms", "fft_ms", "whiten_ms", "harmsum_ms", "finalize_ms", "total_ms")},
         "ingest_s": ingest_s,
         "roofline": {"bound": "valu", "kernel": "k_dedisp_u8", "achieved": ach_tadds, "peak": VALU_PEAK_TADDS,
-                     "unit": "Tadd/s", "frac": ach_tadds / VALU_PEAK_TADDS, "traffic": None},
+                     "unit": "Tadd/s", "frac": ach_tadds / VALU_PEAK_TADDS,
+                     "traffic": dd_traffic * units_per_launch if dd_traffic else None,
+                     "lds": {"achieved": ach_lds, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": ach_lds / LDS_PEAK_GBS},
+                     "note": "adds = trials x series_len x nchans; HIP-event time of the dedispersion stage"},
         "roofline_spectral": {"bound": "hbm", "kernels": "fft+whiten+harmsum",
-                              "achieved": float(units) * (4 + 4 * r) / (spec_ms * 1e-3) / 1e9,
+                              "achieved": float(units) * spec_alg / (spec_ms * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": float(units) * (4 + 4 * r) / (spec_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                              "traffic": None},
+                              "frac": float(units) * spec_alg / (spec_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "traffic": spec_traffic * units_per_launch if spec_traffic else None},
+        "launches_per_step": launches,
     }
+    if not a.no_cpu:
+        res["fp64_check"] = fp64_error(obs, w, n, out16, c16, cap, a.fp64_trials)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(payload, w, n, a.cpu_trials, min(16, os.cpu_count() or 1))
     print(json.dumps(res), flush=True)

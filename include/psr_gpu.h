@@ -105,6 +105,7 @@ typedef struct {
     uint64_t trials;
     uint64_t series_len;
     uint64_t fft_len;
+    uint64_t launches; /* device batches (one dedispersion launch each) */
 } psr_stage_times;
 
 /* ---- library / context ------------------------------------------------- */

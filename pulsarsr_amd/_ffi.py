@@ -94,7 +94,7 @@ class StageTimes(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("delays_ms", C.c_double), ("dedisperse_ms", C.c_double),
                 ("fft_ms", C.c_double), ("whiten_ms", C.c_double), ("harmsum_ms", C.c_double),
                 ("finalize_ms", C.c_double), ("trials", C.c_uint64), ("series_len", C.c_uint64),
-                ("fft_len", C.c_uint64)]
+                ("fft_len", C.c_uint64), ("launches", C.c_uint64)]
 
 
 class SynthPulsar(C.Structure):

@@ -682,6 +682,7 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
     c->last.trials = ntrials;
     c->last.series_len = n;
     c->last.fft_len = g.nfft;
+    c->last.launches = (ntrials + chunk - 1) / chunk;
     if (overflow)
         return fail(PSR_E_OVERFLOW, std::to_string(overflow) +
                                         " trial(s) produced more peaks than the per-trial buffer holds");

@@ -194,6 +194,11 @@ __device__ __forceinline__ uint32_t load_u32(const uint8_t* row, uint64_t off, u
     return off + 4 <= pitch ? *(const uint32_t*)(row + off) : 0u;
 }
 
+// pair word of byte i (0..3) of windows a (low half) and b (high half)
+__device__ __forceinline__ uint32_t pair_word(uint32_t b, uint32_t a, int i) {
+    return __builtin_amdgcn_perm(b, a, 0x0c040c00u + 0x00010001u * (uint32_t)i);
+}
+
 __global__ __launch_bounds__(256) void k_dedisp_u8(
     const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_chans,
     const int32_t* __restrict__ bases, const int32_t* __restrict__ rel, uint32_t ntrials,
@@ -228,28 +233,36 @@ __global__ __launch_bounds__(256) void k_dedisp_u8(
     }
     uint32_t in_phase = 0;
     const uint32_t quads = W4 / 4;
+    const uint32_t c0 = threadIdx.x / quads, q0 = threadIdx.x % quads;
+    const uint32_t sc = blockDim.x / quads, sq = blockDim.x % quads;
     const char* lds_lane = (const char*)lds + 16 * lane;
     for (uint32_t cc = 0; cc < nchans; cc += C) {
         const uint32_t nc = min(C, nchans - cc);
         __syncthreads();
+        // item i = (channel c, quad qd), walked without division: (c0, q0) is the
+        // thread's first item, each step adds (sc, sq)
+        uint32_t c = c0, qd = q0;
         for (uint32_t i = threadIdx.x; i < nc * quads; i += blockDim.x) {
-            const uint32_t c = i / quads, qd = i - c * quads;
             const uint8_t* row = data + (size_t)(cc + c) * pitch;
             const uint64_t off = (uint64_t)t0 + (uint32_t)bblk[cc + c] + 4u * qd;
             const uint32_t a0 = load_u32(row, off, pitch), a1 = load_u32(row, off + 4, pitch);
             const uint32_t b0 = load_u32(row, off + 256, pitch), b1 = load_u32(row, off + 260, pitch);
             uint32_t* dst = lds + (size_t)c * 4 * W4 + 4 * qd;
+            // w[j] pairs byte j of the 8-byte window at off with byte j of the window
+            // at off + 256; copy k holds w[k .. k + 3]
+            uint32_t w[7];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                // bytes k..k+3 of the 8-byte windows starting at off and off + 256
-                const uint32_t a = __builtin_amdgcn_alignbyte(a1, a0, k);
-                const uint32_t b = __builtin_amdgcn_alignbyte(b1, b0, k);
-                uint4 w;
-                w.x = __builtin_amdgcn_perm(b, a, 0x0c040c00u);
-                w.y = __builtin_amdgcn_perm(b, a, 0x0c050c01u);
-                w.z = __builtin_amdgcn_perm(b, a, 0x0c060c02u);
-                w.w = __builtin_amdgcn_perm(b, a, 0x0c070c03u);
-                *(uint4*)(dst + (size_t)k * W4) = w;
+            for (int j = 0; j < 4; j++) w[j] = pair_word(b0, a0, j);
+#pragma unroll
+            for (int j = 0; j < 3; j++) w[4 + j] = pair_word(b1, a1, j);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                *(uint4*)(dst + (size_t)k * W4) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+            c += sc;
+            qd += sq;
+            if (qd >= quads) {
+                qd -= quads;
+                c++;
             }
         }
         for (uint32_t i = threadIdx.x; i < nc * DD_TRIALS; i += blockDim.x) {
