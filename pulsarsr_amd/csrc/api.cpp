@@ -542,7 +542,7 @@ static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const doub
     const double inv_s0 = 1.0 / o->s0;
     bool int_done = false;
     if (o->kind == KIND_U8) {
-        const uint32_t spread = spread_bound(o, dms_host, nt, 32);
+        const uint32_t spread = spread_bound(o, dms_host, nt, DEDISP_BLOCK_TRIALS);
         if ((rc = ensure(c, c->dd_table, dedisp_u8_table_words(o->nchans, nt) * sizeof(int32_t)))) return rc;
         int_done = launch_dedisp_u8(s, (const uint8_t*)o->data, o->pitch, o->nchans, o->maxval,
                                     (const int32_t*)c->delays.p, (int32_t*)c->dd_table.p, nt, n,

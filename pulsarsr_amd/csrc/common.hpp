@@ -66,6 +66,8 @@ void launch_chan_stats(hipStream_t s, const void* data, int kind, uint32_t nchan
 void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double ref_inv,
                    double tsamp, const double* dms, uint32_t ntrials, int32_t* out);
 // integer dedispersion (u8 rows), outputs f32 exact sums or FP16 scaled
+// DM trials per dedispersion workgroup (the delay spread bound is per block)
+constexpr uint32_t DEDISP_BLOCK_TRIALS = 64;
 // Returns false (and launches nothing) when the block's delay spread needs more
 // LDS than a workgroup has; the caller then uses launch_dedisp_f32.
 bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,

@@ -140,8 +140,8 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 }
 
 // ------------------------------------------------- integer dedispersion (u8)
-// Workgroup = 4 waves sharing one time tile of DD_TILE = 512 outputs and a block
-// of DD_TRIALS = 32 consecutive (ascending) DM trials; each wave owns DD_M = 8
+// Workgroup = 8 waves sharing one time tile of DD_TILE = 512 outputs and a block
+// of DD_TRIALS = 64 consecutive (ascending) DM trials; each wave owns DD_M = 8
 // trials.  Channels are staged chunk by chunk into LDS as 32-bit words
 //     w[i] = s[base + i] | s[base + i + 256] << 16,
 // with base = the block's smallest delay of the channel rounded down to 4.  A
@@ -152,8 +152,9 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // starting k words later; the trial reads copy e & 3 at word e & ~3.  The 64
 // lanes of a wave read 1 KiB of consecutive LDS, free of bank conflicts.
 //
-// Per-trial LDS addresses are staged with the rows (no scalar loads inside the
-// channel loop, so LDS waits stay fine-grained).  16-bit accumulators widen to
+// Per-trial LDS byte offsets (16 bits each) are staged with the rows; a wave
+// reads its 8 offsets of a channel with one 16-byte broadcast read, so no scalar
+// load sits among the LDS reads and the waits stay fine-grained.  16-bit accumulators widen to
 // 32 bits every `phase_chans` channels, before a half-word can overflow.
 //
 // Workgroups are numbered XCD-aware: consecutive workgroup ids go to different
@@ -161,8 +162,11 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // (tile, block) work items with the block index fastest -- the channel data of
 // a tile is then fetched into that XCD's L2 once and reused by all DM blocks.
 constexpr int DD_M = 8;
+constexpr int DD_WAVES = 8;
+constexpr int DD_THREADS = 64 * DD_WAVES;
 constexpr int DD_TILE = 512;
-constexpr int DD_TRIALS = 4 * DD_M;
+constexpr int DD_TRIALS = DD_WAVES * DD_M;
+static_assert(DD_TRIALS == (int)DEDISP_BLOCK_TRIALS, "block size shared with the host");
 constexpr int DD_XCDS = 8;
 
 __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
@@ -199,13 +203,14 @@ __device__ __forceinline__ uint32_t pair_word(uint32_t b, uint32_t a, int i) {
     return __builtin_amdgcn_perm(b, a, 0x0c040c00u + 0x00010001u * (uint32_t)i);
 }
 
-__global__ __launch_bounds__(256) void k_dedisp_u8(
+__global__ __launch_bounds__(DD_THREADS) void k_dedisp_u8(
     const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_chans,
     const int32_t* __restrict__ bases, const int32_t* __restrict__ rel, uint32_t ntrials,
     uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out, uint32_t out_pitch,
     int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t W4, uint32_t C) {
     extern __shared__ __align__(16) uint32_t lds[];  // [C][4][W4] words, then [C][32] addresses
-    int32_t* laddr = (int32_t*)(lds + (size_t)C * 4 * W4);
+    // [C][DD_TRIALS] LDS byte offsets (< 64 KiB) of the runs each trial reads
+    uint16_t* laddr = (uint16_t*)(lds + (size_t)C * 4 * W4);
 
     // XCD-aware work index (bijective over total = ntiles * nblk)
     const uint32_t total = ntiles * nblk;
@@ -268,14 +273,15 @@ __global__ __launch_bounds__(256) void k_dedisp_u8(
         for (uint32_t i = threadIdx.x; i < nc * DD_TRIALS; i += blockDim.x) {
             const uint32_t c = i / DD_TRIALS;
             const int32_t e = rblk[(size_t)cc * DD_TRIALS + i];
-            laddr[i] = (int32_t)((((size_t)c * 4 + (e & 3)) * W4 + (uint32_t)(e & ~3)) * 4);
+            laddr[i] = (uint16_t)((((uint32_t)c * 4 + (e & 3)) * W4 + (uint32_t)(e & ~3)) * 4);
         }
         __syncthreads();
 #pragma unroll 2
         for (uint32_t c = 0; c < nc; c++) {
-            const int4* ap = (const int4*)(laddr + c * DD_TRIALS + wave * DD_M);
-            const int4 A0 = ap[0], A1 = ap[1];
-            const int addr[DD_M] = {A0.x, A0.y, A0.z, A0.w, A1.x, A1.y, A1.z, A1.w};
+            // the wave's 8 offsets: one 16-byte broadcast read, 16 bits each
+            const uint4 A = *(const uint4*)(laddr + c * DD_TRIALS + wave * DD_M);
+            const uint32_t addr[DD_M] = {A.x & 0xffffu, A.x >> 16, A.y & 0xffffu, A.y >> 16,
+                                         A.z & 0xffffu, A.z >> 16, A.w & 0xffffu, A.w >> 16};
 #pragma unroll
             for (int m = 0; m < DD_M; m++) {
                 const uint4 v = *(const uint4*)(lds_lane + addr[m]);
@@ -361,9 +367,10 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
                       double* partial, uint32_t spread_max) {
     // a trial reads words (e & ~3) .. (e & ~3) + 255 with e <= spread_max + 3
     const uint64_t W4 = ((uint64_t)spread_max + 3 + 3) / 4 * 4 + DD_TILE / 2;
-    const uint64_t per_chan = 16 * W4 + 4 * DD_TRIALS;  // four row copies + addresses
-    uint64_t budget = 40 * 1024;
-    if (per_chan > budget) budget = 150 * 1024;
+    // four row copies + 16-bit offsets per channel; the row copies must stay below
+    // 64 KiB (16-bit offsets), which also lets two 512-thread workgroups share a CU
+    const uint64_t per_chan = 16 * W4 + 2 * DD_TRIALS;
+    const uint64_t budget = 64 * 1024;
     if (per_chan > budget) return false;
     uint32_t C = (uint32_t)std::min<uint64_t>(budget / per_chan, 64);
     C = std::min(C, nchans);
@@ -380,7 +387,7 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     const uint32_t ntiles = (n + DD_TILE - 1) / DD_TILE;
     const uint64_t total = (uint64_t)ntiles * nblk;
-    hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(256), shm, s, data, pitch, nchans,
+    hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(DD_THREADS), shm, s, data, pitch, nchans,
                        phase, bases, rel, ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16,
                        m0, inv_s0, partial, (uint32_t)W4, C);
     return true;
