@@ -145,6 +145,19 @@ void psr_obs_destroy(psr_obs *obs);
 /* Copy the channel-major f32 samples back (nchans*nsamples floats), for checks. */
 int psr_obs_download_f32(psr_obs *obs, float *out);
 
+/* rfi::Mask (crates/rfi/src/lib.rs:23-29); the channel and block lists go to
+ * caller arrays. */
+typedef struct {
+    uint64_t nchannels, nblocks, block_samples;
+    double masked_fraction;
+} psr_mask;
+/* rfi::mask (crates/rfi/src/lib.rs:31-96) applied in place to a device-resident
+ * observation: masked samples take their channel's median, bit for bit as the
+ * reference does on its f32 array.  channels[nchans] and
+ * blocks[ceil(nsamples / block_samples)] receive the masked indices. */
+int psr_obs_mask(psr_ctx *ctx, psr_obs *obs, uint64_t block_samples, float threshold,
+                 psr_mask *out, uint64_t *channels, uint64_t *blocks);
+
 /* ---- the hot path ------------------------------------------------------- */
 /* dedisperse::delays + dedisperse::apply for `ntrials` DMs
  * (crates/dedisperse/src/lib.rs:9-34, 62-90).  length==0 means None (full

@@ -90,6 +90,11 @@ class Outcome(C.Structure):  # psr_outcome
                 ("ncandidates", C.c_uint64)]
 
 
+class Mask(C.Structure):  # psr_mask
+    _fields_ = [("nchannels", C.c_uint64), ("nblocks", C.c_uint64), ("block_samples", C.c_uint64),
+                ("masked_fraction", C.c_double)]
+
+
 class StageTimes(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("delays_ms", C.c_double), ("dedisperse_ms", C.c_double),
                 ("fft_ms", C.c_double), ("whiten_ms", C.c_double), ("harmsum_ms", C.c_double),
@@ -130,6 +135,8 @@ EXPORTS = [
     ("psr_obs_create_f32", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)]),
     ("psr_obs_destroy", None, [C.c_void_p]),
     ("psr_obs_download_f32", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("psr_obs_mask", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_float, C.POINTER(Mask),
+                               C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("psr_dedisperse", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
     ("psr_search", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t, C.POINTER(C.c_uint32)]),
     ("psr_last_stage_times", C.c_int, [C.c_void_p, C.POINTER(StageTimes)]),

@@ -351,8 +351,14 @@ static int obs_finish(psr_ctx* c, psr_obs* o) {
     for (uint32_t ch = 0; ch < o->nchans; ch++) inv[ch] = 1.0 / (o->freqs[ch] * o->freqs[ch]);
     HIP_TRY(hipMalloc((void**)&o->inv_f2, o->nchans * sizeof(double)));
     HIP_TRY(hipMemcpy(o->inv_f2, inv.data(), o->nchans * sizeof(double), hipMemcpyHostToDevice));
-    // FP16 storage offset/scale: sum of channel means (rounded to an integer for
-    // integer samples) and sqrt of the summed channel variances.
+    return obs_storage_scale(c, o);
+}
+
+}  // extern "C"
+
+// FP16 storage offset/scale of an observation: sum of channel means (rounded to
+// an integer for integer samples) and sqrt of the summed channel variances.
+int psr::obs_storage_scale(psr_ctx* c, psr_obs* o) {
     double *dsum = nullptr, *dsq = nullptr;
     HIP_TRY(hipMalloc((void**)&dsum, o->nchans * sizeof(double)));
     HIP_TRY(hipMalloc((void**)&dsq, o->nchans * sizeof(double)));
@@ -373,6 +379,8 @@ static int obs_finish(psr_ctx* c, psr_obs* o) {
     o->s0 = v > 1e-12 ? std::sqrt(v) : 1.0;
     return PSR_OK;
 }
+
+extern "C" {
 
 int psr_obs_create(psr_ctx* c, const psr_filterbank* fb, psr_obs** out) {
     clear_error();

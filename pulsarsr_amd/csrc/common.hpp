@@ -115,6 +115,19 @@ void launch_trial_mean(hipStream_t s, const double* partial, uint32_t ntiles, ui
 void launch_fold(hipStream_t s, const float* series, uint64_t n, double tsamp,
                  const double* periods, const uint64_t* offsets, uint32_t nprof, uint32_t bins,
                  double* sums, uint64_t* counts);
+// RFI mask (crates/rfi/src/lib.rs:31-96) on a device observation
+void launch_rfi_row_stats(hipStream_t s, const void* data, int kind, uint64_t pitch, uint32_t nch,
+                          uint64_t n, float* means, float* vars);
+void launch_rfi_block_sums(hipStream_t s, const void* data, int kind, uint64_t pitch, uint32_t nch,
+                           uint64_t n, uint32_t block, uint32_t nblocks, float* sums);
+void launch_rfi_block_means(hipStream_t s, const float* sums, const uint8_t* bad, uint32_t nch,
+                            uint64_t n, uint32_t block, uint32_t nblocks, uint32_t active,
+                            float* means);
+void launch_rfi_median(hipStream_t s, const void* data, int kind, uint64_t pitch, uint32_t nch,
+                       uint64_t n, uint32_t* median_key);
+void launch_rfi_fill(hipStream_t s, void* data, int kind, uint64_t pitch, uint32_t nch, uint64_t n,
+                     uint32_t block, const uint8_t* bad, const uint8_t* bad_blocks,
+                     const uint32_t* median_key);
 // confirmation stage: per-channel profiles of raw rows and the DM response
 void launch_fold_rows(hipStream_t s, const void* data, int kind, uint64_t pitch, uint32_t nrows,
                       uint64_t n, double tsamp, double period, uint32_t bins, double* sums,

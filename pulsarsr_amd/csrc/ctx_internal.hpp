@@ -62,6 +62,8 @@ int check_device(psr_ctx* c);
 // rows out[i * n ..]; the caller has validated n against the delays.
 int dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t n,
                        float* out);
+// recompute the FP16 storage offset/scale (m0, s0) from the device samples
+int obs_storage_scale(psr_ctx* c, psr_obs* o);
 // bytes per sample of an observation's channel rows
 inline size_t sample_bytes(const psr_obs* o) {
     return o->kind == KIND_U8 ? 1 : (o->kind == KIND_U16 ? 2 : 4);
