@@ -84,6 +84,15 @@ def confirmation(report, families=2, dm_every=8):
     return dict(outcome=head, rules=c["rules"], families=out)
 
 
+def skeleton(v):
+    """Key structure of a JSON value: dicts keep key order, lists keep their first element's shape."""
+    if isinstance(v, dict):
+        return {k: skeleton(x) for k, x in v.items()}
+    if isinstance(v, list):
+        return [skeleton(v[0])] if v else []
+    return type(v).__name__
+
+
 def main():
     gbt = header(os.path.join(REF, "GBT_Lband_PSR.fil"))
     syn = header(os.path.join(REF, "synthetic.fil"))
@@ -104,6 +113,8 @@ def main():
               open(os.path.join(OUT, "gbt_confirm.json"), "w"), separators=(",", ":"))
     json.dump(confirmation(os.path.join(REF, "output_confirm_synth/report.json")),
               open(os.path.join(OUT, "synthetic_confirm.json"), "w"), separators=(",", ":"))
+    json.dump(skeleton(json.load(open(os.path.join(REF, "output_confirm_gbt/report.json")))),
+              open(os.path.join(OUT, "report_schema.json"), "w"), indent=1)
 
 
 if __name__ == "__main__":
