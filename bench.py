@@ -23,10 +23,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# MI355X: 256 CUs at 2.4 GHz.  VALU: 64 lanes per CU per clock; v_pk_add_u16
-# (the dedispersion add) does two 16-bit adds per lane.  LDS: 128 B per CU per clock.
-VALU_PEAK_TADDS = 256 * 64 * 2 * 2.4e9 / 1e12
-LDS_PEAK_GBS = 256 * 128 * 2.4e9 / 1e9
+# MI355X: 256 CUs at 2.4 GHz.  LDS: 256 B per CU per clock of ds_read_b128 (a
+# probe limited by its own VALU reached 215 B/clk/CU).  VALU: 64 lanes per CU per
+# clock; v_add3_u32 on u16 pairs (the dedispersion add) does four 16-bit adds per
+# lane.  Each dedispersion add reads a 2-byte operand from LDS, so LDS binds first.
+LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9
+VALU_PEAK_TADDS = 256 * 64 * 4 * 2.4e9 / 1e12
 
 
 def log(*a):
@@ -231,11 +233,13 @@ def main():
         "top20_max_rel_snr_diff_fp16_vs_fp32": snr_err,
         "stages_ms": {k: stages[k] for k in ("delays_ms", "dedisperse_ms", "fft_ms", "whiten_ms", "harmsum_ms", "finalize_ms", "total_ms")},
         "ingest_s": ingest_s,
-        "roofline": {"bound": "valu", "kernel": "k_dedisp_u8", "achieved": ach_tadds, "peak": VALU_PEAK_TADDS,
-                     "unit": "Tadd/s", "frac": ach_tadds / VALU_PEAK_TADDS,
+        "roofline": {"bound": "lds", "kernel": "k_dedisp_u8", "achieved": ach_lds, "peak": LDS_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach_lds / LDS_PEAK_GBS,
                      "traffic": dd_traffic * units_per_launch if dd_traffic else None,
-                     "lds": {"achieved": ach_lds, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": ach_lds / LDS_PEAK_GBS},
-                     "note": "adds = trials x series_len x nchans; HIP-event time of the dedispersion stage"},
+                     "valu": {"achieved": ach_tadds, "peak": VALU_PEAK_TADDS, "unit": "Tadd/s",
+                              "frac": ach_tadds / VALU_PEAK_TADDS},
+                     "note": "2 LDS bytes per add, adds = trials x series_len x nchans; HIP-event time of "
+                             "the dedispersion stage; traffic = HBM bytes per launch"},
         "roofline_spectral": {"bound": "hbm", "kernels": "fft+whiten+harmsum",
                               "achieved": float(units) * spec_alg / (spec_ms * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",

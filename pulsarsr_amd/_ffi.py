@@ -20,6 +20,7 @@ PSR_E_CONTRACT = -6
 PSR_E_OVERFLOW = -7
 PRECISION_FP32 = 0
 PRECISION_FP16 = 1
+PSR_CTX_OVERLAP = 1
 
 
 class Candidate(C.Structure):

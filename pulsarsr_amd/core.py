@@ -20,11 +20,17 @@ def check(rc):
 
 
 class Context:
-    """A psr_ctx: one HIP stream and the device buffers reused across calls."""
+    """A psr_ctx: its HIP streams and the device buffers reused across calls.
 
-    def __init__(self, device=0):
+    ``overlap=True`` (PSR_CTX_OVERLAP) makes psr_search dedisperse the next
+    trial batch on a second stream while the current batch's spectral stages
+    run; by default batches run one after another and each reported stage time
+    is that stage's own kernel time.
+    """
+
+    def __init__(self, device=0, overlap=False):
         self._h = C.c_void_p()
-        check(_ffi.lib().psr_ctx_create(device, 0, C.byref(self._h)))
+        check(_ffi.lib().psr_ctx_create(device, _ffi.PSR_CTX_OVERLAP if overlap else 0, C.byref(self._h)))
 
     @property
     def handle(self):

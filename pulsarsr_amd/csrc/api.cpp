@@ -83,15 +83,23 @@ int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t l
 // Upper bound of the within-block delay spread of any channel: delays are
 // round(K*dm*a_c/tsamp) with a_c <= a_max, so the spread over a block of DMs is
 // at most K*a_max*(dm_hi - dm_lo)/tsamp + 1; two samples of slack on top.
-uint32_t spread_bound(const psr_obs* o, const double* dms_sorted, size_t ntrials, size_t block) {
-    double amax = 0.0;
-    for (double f : o->freqs) amax = std::max(amax, 1.0 / (f * f) - o->ref_inv);
+// Per-channel bound on the delay spread (samples) over any block of `block`
+// consecutive sorted trials: the widest DM span of a block times the channel's
+// dispersion slope, plus 3 samples for rounding.
+std::vector<uint32_t> spread_bounds(const psr_obs* o, const double* dms_sorted, size_t ntrials,
+                                    size_t block) {
     double worst = 0.0;
     for (size_t b = 0; b < ntrials; b += block) {
         const size_t e = std::min(ntrials, b + block) - 1;
         worst = std::max(worst, dms_sorted[e] - dms_sorted[b]);
     }
-    return (uint32_t)std::ceil(DISPERSION_SECONDS_MHZ * amax * worst / o->tsamp) + 3u;
+    std::vector<uint32_t> out(o->freqs.size());
+    for (size_t c = 0; c < out.size(); c++) {
+        const double f = o->freqs[c];
+        const double a = std::max(0.0, 1.0 / (f * f) - o->ref_inv);
+        out[c] = (uint32_t)std::ceil(DISPERSION_SECONDS_MHZ * a * worst / o->tsamp) + 3u;
+    }
+    return out;
 }
 
 int upload_twiddles(psr_ctx* c, const SpecGeom& g) {
@@ -181,13 +189,27 @@ int make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, Sear
     return PSR_OK;
 }
 
-// Run the spectral chain (FFT, whitening, harmonic sums, ranking) over `nt`
-// series already in c->series (and their means/corrections), writing ranked
-// candidates to host `out`.  Events e1..e5 bracket the stages.
+// A trial-series slot: dedispersed series, their means/corrections and DMs.
+struct Slot {
+    DevBuf* series;
+    DevBuf* mean32;
+    DevBuf* corr16;
+    DevBuf* dms;
+};
+Slot slot_of(psr_ctx* c, int i) {
+    return i == 0 ? Slot{&c->series, &c->mean32, &c->corr16, &c->dms}
+                  : Slot{&c->series_b, &c->mean32_b, &c->corr16_b, &c->dms_b};
+}
+
+// Enqueue the spectral chain (FFT, whitening, harmonic sums, ranking) over `nt`
+// series of slot `sl` on stream `s`, ending with the copy of the ranked
+// candidates to host `out` and of the counts (nt + 1 words, the last one the
+// peak-overflow count) to host `counts`.  Nothing waits: the caller synchronises.
+// With `ev`, events ev[0..4] bracket FFT, whitening, harmonic sums and ranking.
 int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
-                 const std::vector<uint32_t>& birdies, const double* dms_dev, uint32_t nt,
-                 psr_candidate* out, size_t cap_per_trial, uint32_t* counts, bool timed,
-                 uint32_t* overflow) {
+                 const std::vector<uint32_t>& birdies, const Slot& sl, uint32_t nt,
+                 psr_candidate* out, size_t cap_per_trial, uint32_t* counts, hipStream_t s,
+                 hipEvent_t* ev) {
     const size_t es = p.fp16 ? 2 : 4;
     const uint32_t npart = g.m1 == 1 ? 1u : g.m2 / 2 + 1;
     int rc;
@@ -206,40 +228,63 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
         (rc = ensure(c, c->counts, (size_t)(nt + 1) * sizeof(uint32_t))) ||
         (rc = ensure(c, c->birdies, std::max<size_t>(birdies.size(), 1) * sizeof(uint32_t))))
         return rc;
-    hipStream_t s = c->stream;
     std::vector<uint32_t> bh(2 * MAX_BIRDIE_INTERVALS, 0);
     std::copy(birdies.begin(), birdies.end(), bh.begin());
     HIP_TRY(hipMemsetAsync(c->peak_counts.p, 0, (size_t)nt * sizeof(uint32_t), s));
     HIP_TRY(hipMemsetAsync(c->counts.p, 0, (size_t)(nt + 1) * sizeof(uint32_t), s));
-    if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
-    launch_fft(s, g, c->series.p, (int)p.fp16, (const float*)c->mean32.p, (const float*)c->corr16.p,
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s));
+    launch_fft(s, g, sl.series->p, (int)p.fp16, (const float*)sl.mean32->p, (const float*)sl.corr16->p,
                nt, (float2*)c->work.p, c->spec_t.p, (double*)c->fft_partial.p,
                (const float2*)c->twiddles.p, (const float*)c->costabs.p);
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s));
     launch_whiten(s, g, c->spec_t.p, (int)p.fp16, nt, (const double*)c->fft_partial.p, npart,
                   c->raw_nat.p, c->blk_sorted.p, (float*)c->floors.p, (float*)c->scales.p);
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], s));
     launch_harmsum(s, g, p, c->raw_nat.p, (const float*)c->scales.p, bh.data(), nt,
                    (RawPeak*)c->peaks.p, (uint32_t*)c->peak_counts.p);
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(c->ev[5], s));
-    launch_finalize(s, g, p, dms_dev, nt, (const RawPeak*)c->peaks.p,
+    if (ev) HIP_TRY(hipEventRecord(ev[3], s));
+    launch_finalize(s, g, p, (const double*)sl.dms->p, nt, (const RawPeak*)c->peaks.p,
                     (const uint32_t*)c->peak_counts.p, (CandidateOut*)c->cands.p,
                     (uint32_t)cap_per_trial, (uint32_t*)c->counts.p);
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(c->ev[6], s));
-    std::vector<uint32_t> cnt(nt + 1);
-    HIP_TRY(hipMemcpyAsync(cnt.data(), c->counts.p, (size_t)(nt + 1) * sizeof(uint32_t),
+    HIP_TRY(hipMemcpyAsync(counts, c->counts.p, (size_t)(nt + 1) * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(out, c->cands.p, (size_t)nt * cap_per_trial * sizeof(CandidateOut),
                            hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::memcpy(counts, cnt.data(), nt * sizeof(uint32_t));
-    *overflow += cnt[nt];
+    if (ev) HIP_TRY(hipEventRecord(ev[4], s));
     return PSR_OK;
 }
+
+int ensure_pinned(psr_ctx* c, size_t bytes) {
+    if (c->pinned_bytes >= bytes) return PSR_OK;
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_bytes = 0;
+    hipError_t e = hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c->pinned = nullptr;
+        return fail(PSR_E_OOM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+    }
+    c->pinned_bytes = bytes;
+    return PSR_OK;
+}
+
+// hipEvent_t array owned for the duration of one call
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    ~EventSet() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    int create(size_t n) {
+        ev.assign(n, nullptr);
+        for (size_t i = 0; i < n; i++) HIP_TRY(hipEventCreate(&ev[i]));
+        return PSR_OK;
+    }
+};
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
@@ -256,9 +301,9 @@ const char* psr_version(void) { return "pulsarsr_amd 0.1.0 (gfx950)"; }
 const char* psr_last_error(void) { return last_error(); }
 
 int psr_ctx_create(int device, uint32_t flags, psr_ctx** out) {
-    (void)flags;
     clear_error();
     if (!out) return fail(PSR_E_INVALID, "null output pointer");
+    if (flags & ~(uint32_t)PSR_CTX_OVERLAP) return fail(PSR_E_INVALID, "unknown context flags");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev <= 0) return fail(PSR_E_DEVICE, "no HIP device available");
@@ -269,8 +314,10 @@ int psr_ctx_create(int device, uint32_t flags, psr_ctx** out) {
         return fail(PSR_E_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
     psr_ctx* c = new psr_ctx();
     c->device = device;
+    c->flags = flags;
     e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
     for (int i = 0; e == hipSuccess && i < 8; i++) e = hipEventCreate(&c->ev[i]);
     if (e != hipSuccess) {
         delete c;
@@ -284,7 +331,9 @@ void psr_ctx_destroy(psr_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     DevBuf* all[] = {&c->delays, &c->dd_table, &c->dms, &c->series, &c->dd_partial, &c->mean32,
+                     &c->series_b, &c->mean32_b, &c->corr16_b, &c->dms_b,
                      &c->corr16, &c->work, &c->spec_t, &c->raw_nat, &c->scales, &c->blk_sorted,
                      &c->floors, &c->fft_partial, &c->peaks, &c->peak_counts, &c->cands,
                      &c->counts, &c->twiddles, &c->costabs, &c->birdies,
@@ -294,6 +343,8 @@ void psr_ctx_destroy(psr_ctx* c) {
     for (int i = 0; i < 8; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     delete c;
 }
 
@@ -527,46 +578,60 @@ int psr_obs_download_f32(psr_obs* o, float* out) {
 }
 
 // ------------------------------------------------------------ dedispersion
-// Dedisperse `nt` trials (ascending DMs already on the device in c->dms) into
-// c->series at `g.series_pitch`, with per-trial means for the FFT input.
-static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const double* dms_host,
-                           uint32_t nt, int fp16, bool timed) {
-    hipStream_t s = c->stream;
+// Device buffers for dedispersing up to `nt` trials into slot `sl`.
+static int ensure_dedisp(psr_ctx* c, psr_obs* o, const SpecGeom& g, const Slot& sl, uint32_t nt,
+                         int fp16) {
     const size_t es = fp16 ? 2 : 4;
-    const uint32_t n = g.n;
+    const uint32_t ntiles_f32 = (g.n + 255) / 256;
     int rc;
-    const uint32_t ntiles_u8 = (n + 511) / 512, ntiles_f32 = (n + 255) / 256;
     if ((rc = ensure(c, c->delays, (size_t)nt * o->nchans * sizeof(int32_t))) ||
-        (rc = ensure(c, c->series, (size_t)nt * g.series_pitch * es + 64)) ||
+        (rc = ensure(c, *sl.series, (size_t)nt * g.series_pitch * es + 64)) ||
         (rc = ensure(c, c->dd_partial, (size_t)nt * ntiles_f32 * sizeof(double))) ||
-        (rc = ensure(c, c->mean32, (size_t)nt * sizeof(float))) ||
-        (rc = ensure(c, c->corr16, (size_t)nt * sizeof(float))))
+        (rc = ensure(c, *sl.mean32, (size_t)nt * sizeof(float))) ||
+        (rc = ensure(c, *sl.corr16, (size_t)nt * sizeof(float))) ||
+        (rc = ensure(c, *sl.dms, (size_t)nt * sizeof(double))))
         return rc;
-    if (timed) HIP_TRY(hipEventRecord(c->ev[0], s));
-    launch_delays(s, o->inv_f2, o->nchans, o->ref_inv, o->tsamp, (const double*)c->dms.p, nt,
+    if (o->kind == KIND_U8 &&
+        (rc = ensure(c, c->dd_table, dedisp_u8_table_words(o->nchans, nt) * sizeof(int32_t))))
+        return rc;
+    return PSR_OK;
+}
+
+// Enqueue on `s` the dedispersion of `nt` trials (ascending DMs already in
+// sl.dms, the same values in host `dms_host`) into sl.series at
+// `g.series_pitch`, with per-trial means for the FFT input.  Buffers must have
+// been sized by ensure_dedisp.  With `ev`, events ev[0..2] bracket the delay
+// table and the dedispersion.
+static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const Slot& sl,
+                           const double* dms_host, uint32_t nt, int fp16, hipStream_t s,
+                           hipEvent_t* ev) {
+    const uint32_t n = g.n;
+    const uint32_t ntiles_u8 = (n + 511) / 512, ntiles_f32 = (n + 255) / 256;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s));
+    launch_delays(s, o->inv_f2, o->nchans, o->ref_inv, o->tsamp, (const double*)sl.dms->p, nt,
                   (int32_t*)c->delays.p);
     HIP_TRY(hipGetLastError());
-    if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s));
     const double inv_s0 = 1.0 / o->s0;
     bool int_done = false;
     if (o->kind == KIND_U8) {
-        const uint32_t spread = spread_bound(o, dms_host, nt, DEDISP_BLOCK_TRIALS);
-        if ((rc = ensure(c, c->dd_table, dedisp_u8_table_words(o->nchans, nt) * sizeof(int32_t)))) return rc;
+        const std::vector<uint32_t> spread = spread_bounds(o, dms_host, nt, DEDISP_BLOCK_TRIALS);
         int_done = launch_dedisp_u8(s, (const uint8_t*)o->data, o->pitch, o->nchans, o->maxval,
                                     (const int32_t*)c->delays.p, (int32_t*)c->dd_table.p, nt, n,
-                                    c->series.p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
-                                    (double*)c->dd_partial.p, spread);
+                                    sl.series->p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
+                                    (double*)c->dd_partial.p, spread.data());
     }
     if (!int_done) {
         launch_dedisp_f32(s, o->data, o->kind, o->pitch, o->nchans, (const int32_t*)c->delays.p, nt,
-                          n, c->series.p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
+                          n, sl.series->p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
                           (double*)c->dd_partial.p);
     }
     const uint32_t ntiles = int_done ? ntiles_u8 : ntiles_f32;
     HIP_TRY(hipGetLastError());
     launch_trial_mean(s, (const double*)c->dd_partial.p, ntiles, nt, n, o->m0, inv_s0,
-                      (float*)c->mean32.p, (float*)c->corr16.p);
+                      (float*)sl.mean32->p, (float*)sl.corr16->p);
     HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[2], s));
     return PSR_OK;
 }
 
@@ -601,11 +666,12 @@ int psr::dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t nt
     const SpecGeom g = make_geometry((uint32_t)n);
     const size_t chunk = 1024;
     std::vector<float> rowbuf;
+    const Slot sl = slot_of(c, 0);
     for (size_t b = 0; b < ntrials; b += chunk) {
         const uint32_t nt = (uint32_t)std::min(chunk, ntrials - b);
-        if ((rc = ensure(c, c->dms, nt * sizeof(double)))) return rc;
-        HIP_TRY(hipMemcpy(c->dms.p, sorted.data() + b, nt * sizeof(double), hipMemcpyHostToDevice));
-        if ((rc = dedisperse_into(c, o, g, sorted.data() + b, nt, 0, false))) return rc;
+        if ((rc = ensure_dedisp(c, o, g, sl, nt, 0))) return rc;
+        HIP_TRY(hipMemcpy(sl.dms->p, sorted.data() + b, nt * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = dedisperse_into(c, o, g, sl, sorted.data() + b, nt, 0, c->stream, nullptr))) return rc;
         rowbuf.resize((size_t)nt * g.series_pitch);
         HIP_TRY(hipMemcpyAsync(rowbuf.data(), c->series.p, rowbuf.size() * sizeof(float),
                                hipMemcpyDeviceToHost, c->stream));
@@ -646,51 +712,98 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return dms[a] < dms[b]; });
     std::vector<double> sorted(ntrials);
     for (size_t i = 0; i < ntrials; i++) sorted[i] = dms[order[i]];
-    // trials per chunk: keep the chunk's working set near 16 GiB
+    // trials per chunk: keep the working set (two series slots, one set of
+    // spectral buffers) near 16 GiB
     const size_t es = p.fp16 ? 2 : 4;
-    const size_t per_trial = (size_t)g.series_pitch * es +
+    const size_t per_trial = 2 * (size_t)g.series_pitch * es +
                              3 * (size_t)g.spec_pitch * es + (size_t)PEAK_CAP * sizeof(RawPeak) +
                              cap_per_trial * sizeof(CandidateOut) + (size_t)o->nchans * 4 + 4096;
-    size_t chunk = std::max<size_t>(32, ((size_t)16 << 30) / per_trial);
+    // (half the ctx memory cap when one is set); whole 64-trial dedispersion blocks
+    size_t budget = (size_t)16 << 30;
+    if (c->mem_cap) budget = std::min<size_t>(budget, c->mem_cap / 2);
+    size_t chunk = std::max<size_t>(1, budget / per_trial);
+    if (chunk >= DEDISP_BLOCK_TRIALS) chunk -= chunk % DEDISP_BLOCK_TRIALS;
     chunk = std::min(chunk, ntrials);
-    chunk = std::max<size_t>(1, chunk);
-    std::vector<psr_candidate> hbuf(chunk * cap_per_trial);
-    std::vector<uint32_t> hcnt(chunk);
+    const size_t nchunks = (ntrials + chunk - 1) / chunk;
+    const bool serial = (c->flags & PSR_CTX_OVERLAP) == 0;
+    hipStream_t sa = c->stream, sb = serial ? c->stream : c->stream2;
+    // both slots and the spectral buffers are sized for the first (largest)
+    // chunk before anything is queued, so no allocation happens while kernels run
+    const Slot slots[2] = {slot_of(c, 0), slot_of(c, nchunks > 1 && !serial ? 1 : 0)};
+    const uint32_t nt0 = (uint32_t)chunk;
+    if ((rc = ensure_dedisp(c, o, g, slots[0], nt0, (int)p.fp16)) ||
+        (rc = ensure_dedisp(c, o, g, slots[1], nt0, (int)p.fp16)))
+        return rc;
+    // pinned host staging: sorted DMs, then per slot the ranked candidates and counts
+    const size_t cand_bytes = chunk * cap_per_trial * sizeof(psr_candidate);
+    const size_t cnt_bytes = (chunk + 1) * sizeof(uint32_t);
+    const size_t dm_bytes = (ntrials * sizeof(double) + 255) & ~(size_t)255;
+    const size_t slot_bytes = (cand_bytes + cnt_bytes + 255) & ~(size_t)255;
+    if ((rc = ensure_pinned(c, dm_bytes + 2 * slot_bytes))) return rc;
+    double* hdms = (double*)c->pinned;
+    for (size_t i = 0; i < ntrials; i++) hdms[i] = sorted[i];
+    auto hcand = [&](int k) { return (psr_candidate*)((char*)c->pinned + dm_bytes + k * slot_bytes); };
+    auto hcnt = [&](int k) { return (uint32_t*)((char*)hcand(k) + cand_bytes); };
+    // events per slot: [0..2] dedispersion, [3..7] spectral; plus the start
+    EventSet evs;
+    if ((rc = evs.create(17))) return rc;
+    hipEvent_t* ev[2] = {evs.ev.data(), evs.ev.data() + 8};
+    hipEvent_t ev_start = evs.ev[16];
     uint32_t overflow = 0;
     double t_delay = 0, t_dd = 0, t_fft = 0, t_wh = 0, t_hs = 0, t_fin = 0;
-    for (size_t b = 0; b < ntrials; b += chunk) {
+    // wait for chunk k (slot k & 1), add its stage times and scatter its results
+    auto drain = [&](size_t k) -> int {
+        const int si = serial ? 0 : (int)(k & 1);
+        hipEvent_t* e = ev[si];
+        HIP_TRY(hipEventSynchronize(e[7]));
+        t_delay += elapsed(e[0], e[1]);
+        t_dd += elapsed(e[1], e[2]);
+        t_fft += elapsed(e[3], e[4]);
+        t_wh += elapsed(e[4], e[5]);
+        t_hs += elapsed(e[5], e[6]);
+        t_fin += elapsed(e[6], e[7]);
+        const size_t b = k * chunk;
         const uint32_t nt = (uint32_t)std::min(chunk, ntrials - b);
-        if ((rc = ensure(c, c->dms, nt * sizeof(double)))) return rc;
-        HIP_TRY(hipMemcpyAsync(c->dms.p, sorted.data() + b, nt * sizeof(double), hipMemcpyHostToDevice,
-                               c->stream));
-        if ((rc = dedisperse_into(c, o, g, sorted.data() + b, nt, (int)p.fp16, true))) return rc;
-        if ((rc = run_spectral(c, g, p, birdies, (const double*)c->dms.p, nt, hbuf.data(),
-                               cap_per_trial, hcnt.data(), true, &overflow)))
-            return rc;
-        t_delay += elapsed(c->ev[0], c->ev[1]);
-        t_dd += elapsed(c->ev[1], c->ev[2]);
-        t_fft += elapsed(c->ev[2], c->ev[3]);
-        t_wh += elapsed(c->ev[3], c->ev[4]);
-        t_hs += elapsed(c->ev[4], c->ev[5]);
-        t_fin += elapsed(c->ev[5], c->ev[6]);
+        const uint32_t* cn = hcnt(si);
+        const psr_candidate* cd = hcand(si);
+        overflow += cn[nt];
         for (uint32_t i = 0; i < nt; i++) {
             const size_t dst = order[b + i];
-            counts[dst] = hcnt[i];
-            std::memcpy(out + dst * cap_per_trial, hbuf.data() + (size_t)i * cap_per_trial,
-                        hcnt[i] * sizeof(psr_candidate));
+            counts[dst] = cn[i];
+            std::memcpy(out + dst * cap_per_trial, cd + (size_t)i * cap_per_trial,
+                        cn[i] * sizeof(psr_candidate));
         }
+        return PSR_OK;
+    };
+    HIP_TRY(hipEventRecord(ev_start, sa));
+    for (size_t k = 0; k < nchunks; k++) {
+        const int si = serial ? 0 : (int)(k & 1);
+        // the slot's previous user (chunk k - 2, or k - 1 when serial) must be done
+        if (serial && k >= 1 && (rc = drain(k - 1))) return rc;
+        if (!serial && k >= 2 && (rc = drain(k - 2))) return rc;
+        const size_t b = k * chunk;
+        const uint32_t nt = (uint32_t)std::min(chunk, ntrials - b);
+        const Slot& sl = slots[si];
+        HIP_TRY(hipMemcpyAsync(sl.dms->p, hdms + b, nt * sizeof(double), hipMemcpyHostToDevice, sa));
+        if ((rc = dedisperse_into(c, o, g, sl, sorted.data() + b, nt, (int)p.fp16, sa, ev[si]))) return rc;
+        if (sb != sa) HIP_TRY(hipStreamWaitEvent(sb, ev[si][2], 0));
+        if ((rc = run_spectral(c, g, p, birdies, sl, nt, hcand(si), cap_per_trial, hcnt(si), sb,
+                               ev[si] + 3)))
+            return rc;
     }
+    for (size_t k = nchunks >= 2 && !serial ? nchunks - 2 : nchunks - 1; k < nchunks; k++)
+        if ((rc = drain(k))) return rc;
     c->last.delays_ms = t_delay;
     c->last.dedisperse_ms = t_dd;
     c->last.fft_ms = t_fft;
     c->last.whiten_ms = t_wh;
     c->last.harmsum_ms = t_hs;
     c->last.finalize_ms = t_fin;
-    c->last.total_ms = t_delay + t_dd + t_fft + t_wh + t_hs + t_fin;
+    c->last.total_ms = elapsed(ev_start, ev[serial ? 0 : (int)((nchunks - 1) & 1)][7]);
     c->last.trials = ntrials;
     c->last.series_len = n;
     c->last.fft_len = g.nfft;
-    c->last.launches = (ntrials + chunk - 1) / chunk;
+    c->last.launches = nchunks;
     if (overflow)
         return fail(PSR_E_OVERFLOW, std::to_string(overflow) +
                                         " trial(s) produced more peaks than the per-trial buffer holds");
@@ -739,12 +852,15 @@ int psr_periodicity(psr_ctx* c, const float* series, size_t n, double tsamp, dou
     if ((rc = make_params(g, tsamp, &cf, &p, &birdies))) return rc;
     if ((rc = upload_series(c, series, n, g))) return rc;
     HIP_TRY(hipMemcpy(c->dms.p, &dm, sizeof(double), hipMemcpyHostToDevice));
-    uint32_t cnt = 0, overflow = 0;
-    if ((rc = run_spectral(c, g, p, birdies, (const double*)c->dms.p, 1, out, cap, &cnt, false,
-                           &overflow)))
+    std::vector<psr_candidate> hout(cap);
+    uint32_t cnt[2] = {0, 0};
+    if ((rc = run_spectral(c, g, p, birdies, slot_of(c, 0), 1, hout.data(), cap, cnt, c->stream,
+                           nullptr)))
         return rc;
-    *count = cnt;
-    if (overflow) return fail(PSR_E_OVERFLOW, "peak buffer overflow");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(out, hout.data(), cnt[0] * sizeof(psr_candidate));
+    *count = cnt[0];
+    if (cnt[1]) return fail(PSR_E_OVERFLOW, "peak buffer overflow");
     return PSR_OK;
 }
 
@@ -770,10 +886,10 @@ int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp,
     const double zero = 0.0;
     HIP_TRY(hipMemcpy(c->dms.p, &zero, sizeof(double), hipMemcpyHostToDevice));
     psr_candidate dummy[1];
-    uint32_t cnt = 0, overflow = 0;
-    if ((rc = run_spectral(c, g, p, birdies, (const double*)c->dms.p, 1, dummy, 1, &cnt, false,
-                           &overflow)))
+    uint32_t cnt[2] = {0, 0};
+    if ((rc = run_spectral(c, g, p, birdies, slot_of(c, 0), 1, dummy, 1, cnt, c->stream, nullptr)))
         return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
     std::vector<float> r(g.nbins), sc(g.nblocks);
     HIP_TRY(hipMemcpy(r.data(), c->raw_nat.p, g.nbins * sizeof(float), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(sc.data(), c->scales.p, g.nblocks * sizeof(float), hipMemcpyDeviceToHost));

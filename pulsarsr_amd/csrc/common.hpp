@@ -54,6 +54,21 @@ struct SearchParams {
     uint32_t peak_cap;  // raw peak capacity per trial
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// xcd_block() maps a 2-D grid's dispatch order to work coordinates so that each
+// XCD walks a contiguous run of work items (x fastest): neighbouring items that
+// share cache lines then run on one XCD at about the same time.  Bijective over
+// the grid for any size.
+constexpr uint32_t NUM_XCDS = 8;
+__device__ inline uint2 xcd_block() {
+    const uint32_t gx = gridDim.x, total = gx * gridDim.y;
+    const uint32_t id = blockIdx.y * gx + blockIdx.x;
+    const uint32_t xcd = id % NUM_XCDS, local = id / NUM_XCDS;
+    const uint32_t q = total / NUM_XCDS, r = total % NUM_XCDS;
+    const uint32_t work = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
+    return make_uint2(work % gx, work / gx);
+}
+
 constexpr int MAX_BIRDIE_INTERVALS = 64;
 constexpr uint32_t PEAK_CAP = 4096;
 
@@ -68,12 +83,14 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // integer dedispersion (u8 rows), outputs f32 exact sums or FP16 scaled
 // DM trials per dedispersion workgroup (the delay spread bound is per block)
 constexpr uint32_t DEDISP_BLOCK_TRIALS = 64;
-// Returns false (and launches nothing) when the block's delay spread needs more
-// LDS than a workgroup has; the caller then uses launch_dedisp_f32.
+// spread[c] (host, per channel) bounds the delay spread of channel c over any
+// block of DEDISP_BLOCK_TRIALS trials.  Returns false (and launches nothing)
+// when the spreads need more LDS than a workgroup has or the totals could pass
+// 2^20; the caller then uses launch_dedisp_f32.
 bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
-                      double* partial, uint32_t spread_max);
+                      double* partial, const uint32_t* spread);
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials);
 // sequential-order f32 dedispersion (16-bit and float rows)
 void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch,

@@ -27,6 +27,7 @@ struct DevBuf {
 
 struct psr_ctx {
     int device = 0;
+    uint32_t flags = 0;
     hipStream_t stream = nullptr;
     uint64_t mem_cap = 0;
     uint64_t mem_used = 0;
@@ -34,6 +35,12 @@ struct psr_ctx {
         scales, blk_sorted, floors, fft_partial, peaks, peak_counts, cands, counts, twiddles, costabs,
         birdies, fold_series, fold_per, fold_off, fold_sums, fold_counts, an_values, an_means,
         an_resp;
+    // second trial-series slot and stream: psr_search dedisperses chunk k+1 on
+    // `stream` while chunk k's spectral stages run on `stream2`
+    hipStream_t stream2 = nullptr;
+    psr::DevBuf series_b, mean32_b, corr16_b, dms_b;
+    void* pinned = nullptr;  // page-locked host staging (candidates, counts, DMs)
+    size_t pinned_bytes = 0;
     psr::SpecGeom tw_geom{};
     bool tw_valid = false;
     hipEvent_t ev[8] = {};

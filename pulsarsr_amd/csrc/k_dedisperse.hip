@@ -7,6 +7,7 @@
 // 32 bits before any half-word can overflow.  16-bit and float samples use a
 // kernel that adds in channel order in f32, exactly as the reference does.
 #include <algorithm>
+#include <vector>
 
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -142,20 +143,28 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // ------------------------------------------------- integer dedispersion (u8)
 // Workgroup = 8 waves sharing one time tile of DD_TILE = 512 outputs and a block
 // of DD_TRIALS = 64 consecutive (ascending) DM trials; each wave owns DD_M = 8
-// trials.  Channels are staged chunk by chunk into LDS as 32-bit words
+// trials.  Channels are staged a few at a time into LDS as 32-bit words
 //     w[i] = s[base + i] | s[base + i + 256] << 16,
 // with base = the block's smallest delay of the channel rounded down to 4.  A
 // trial whose delay is base + e needs words e .. e + 255; lane l takes words
-// e + 4l .. e + 4l + 3 with one 16-byte ds_read_b128 and adds them with four
-// v_pk_add_u16 (8 outputs: 4l + j and 256 + 4l + j).  ds_read_b128 needs a
+// e + 4l .. e + 4l + 3 with one 16-byte ds_read_b128.  ds_read_b128 needs a
 // 16-byte aligned address, so every channel row is staged four times, copy k
 // starting k words later; the trial reads copy e & 3 at word e & ~3.  The 64
-// lanes of a wave read 1 KiB of consecutive LDS, free of bank conflicts.
+// lanes of a wave read 1 KiB of consecutive LDS, free of bank conflicts.  Rows
+// are W4_c words long, from channel c's own delay spread over the block (low
+// frequencies spread most), so a stage holds as many channels as fit.
 //
-// Per-trial LDS byte offsets (16 bits each) are staged with the rows; a wave
-// reads its 8 offsets of a channel with one 16-byte broadcast read, so no scalar
-// load sits among the LDS reads and the waits stay fine-grained.  16-bit accumulators widen to
-// 32 bits every `phase_chans` channels, before a half-word can overflow.
+// Channels are added in pairs with v_add3_u32 on the u16 pairs: within a
+// "phase" of at most 65535 / maxval channels no half can carry, so the 32-bit
+// add is two exact 16-bit adds.  At each phase end the halves are added into
+// running sums mod 2^16 whose carries are counted in 4-bit fields, so totals up
+// to 2^20 are exact.
+//
+// Staging is pipelined: while a stage is summed, each wave loads the raw bytes
+// and trial offsets of one channel of the next stage straight into LDS (MUBUF
+// loads with the LDS bit, no registers held); at the stage boundary the wave
+// turns them into the four row copies and the per-trial LDS offsets (16 bits
+// each; a wave reads its 8 offsets of a channel with one 16-byte broadcast read).
 //
 // Workgroups are numbered XCD-aware: consecutive workgroup ids go to different
 // XCDs, so id is remapped so that each XCD walks a contiguous run of
@@ -168,6 +177,11 @@ constexpr int DD_TILE = 512;
 constexpr int DD_TRIALS = DD_WAVES * DD_M;
 static_assert(DD_TRIALS == (int)DEDISP_BLOCK_TRIALS, "block size shared with the host");
 constexpr int DD_XCDS = 8;
+constexpr int DD_AHEAD = 4;  // trials whose LDS reads are issued ahead of their adds
+constexpr int DD_ROW_BYTES = 56 * 1024;  // row copies of one stage (at most)
+constexpr int DD_RAW_BYTES = 8192;       // raw byte windows of one stage (at most)
+constexpr int DD_RAW_CHANS = DD_WAVES;   // channels per stage (at most): one per wave
+constexpr int DD_RSRC_WORD3 = 0x00020000;  // buffer descriptor word 3: raw 32-bit data
 
 __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
     u16x2 x = __builtin_bit_cast(u16x2, a), y = __builtin_bit_cast(u16x2, b);
@@ -203,14 +217,27 @@ __device__ __forceinline__ uint32_t pair_word(uint32_t b, uint32_t a, int i) {
     return __builtin_amdgcn_perm(b, a, 0x0c040c00u + 0x00010001u * (uint32_t)i);
 }
 
-__global__ __launch_bounds__(DD_THREADS) void k_dedisp_u8(
+__global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_chans,
     const int32_t* __restrict__ bases, const int32_t* __restrict__ rel, uint32_t ntrials,
     uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out, uint32_t out_pitch,
-    int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t W4, uint32_t C) {
-    extern __shared__ __align__(16) uint32_t lds[];  // [C][4][W4] words, then [C][32] addresses
-    // [C][DD_TRIALS] LDS byte offsets (< 64 KiB) of the runs each trial reads
-    uint16_t* laddr = (uint16_t*)(lds + (size_t)C * 4 * W4);
+    int fp16, float m0, float inv_s0, double* __restrict__ partial,
+    const uint32_t* __restrict__ cinfo, const uint32_t* __restrict__ sfirst, uint32_t nstages,
+    uint32_t RB) {
+    // Static LDS: the stage's row copies, the [stage channel][DD_TRIALS] LDS byte
+    // offsets of the runs each trial reads, and the next stage's raw bytes
+    // [channel][RB] and trial offsets [channel][DD_TRIALS] written by direct
+    // global->LDS loads.  They are separate LDS objects, so the compiler can tell
+    // that the sums' reads of the rows need not wait for those loads.  Dynamic
+    // LDS: per channel its base, its row length W4 and row offset (cinfo), then
+    // the stages' first channels.
+    __shared__ __align__(16) uint32_t lds[DD_ROW_BYTES / 4];
+    __shared__ __align__(16) uint16_t laddr[DD_RAW_CHANS * DD_TRIALS];
+    __shared__ __align__(16) uint8_t raw[DD_RAW_BYTES];
+    __shared__ __align__(16) int32_t rrel[DD_RAW_CHANS * DD_TRIALS];
+    extern __shared__ __align__(16) int32_t lbase[];
+    uint32_t* lcinfo = (uint32_t*)(lbase + nchans);
+    uint32_t* lsfirst = lcinfo + nchans;
 
     // XCD-aware work index (bijective over total = ntiles * nblk)
     const uint32_t total = ntiles * nblk;
@@ -227,88 +254,173 @@ __global__ __launch_bounds__(DD_THREADS) void k_dedisp_u8(
     const int32_t* bblk = bases + (size_t)blk * nchans;
     const int32_t* rblk = rel + (size_t)blk * nchans * DD_TRIALS;
 
-    uint32_t acc16[DD_M][4];
-    uint32_t acc32[DD_M][8];
+    // acc16: this phase's sums (u16 pairs, no half can overflow within a phase);
+    // lo: running sums mod 2^16 per half; cy: carries out of each half, 4 bits per
+    // output (bits 4j + 16h for word j, half h; the host keeps every total below 2^20)
+    uint32_t acc16[DD_M][4], lo[DD_M][4], cy[DD_M];
 #pragma unroll
     for (int m = 0; m < DD_M; m++) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc16[m][j] = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) acc32[m][j] = 0;
+        for (int j = 0; j < 4; j++) acc16[m][j] = lo[m][j] = 0;
+        cy[m] = 0;
     }
-    uint32_t in_phase = 0;
-    const uint32_t quads = W4 / 4;
-    const uint32_t c0 = threadIdx.x / quads, q0 = threadIdx.x % quads;
-    const uint32_t sc = blockDim.x / quads, sq = blockDim.x % quads;
-    const char* lds_lane = (const char*)lds + 16 * lane;
-    for (uint32_t cc = 0; cc < nchans; cc += C) {
-        const uint32_t nc = min(C, nchans - cc);
-        __syncthreads();
-        // item i = (channel c, quad qd), walked without division: (c0, q0) is the
-        // thread's first item, each step adds (sc, sq)
-        uint32_t c = c0, qd = q0;
-        for (uint32_t i = threadIdx.x; i < nc * quads; i += blockDim.x) {
-            const uint8_t* row = data + (size_t)(cc + c) * pitch;
-            const uint64_t off = (uint64_t)t0 + (uint32_t)bblk[cc + c] + 4u * qd;
-            const uint32_t a0 = load_u32(row, off, pitch), a1 = load_u32(row, off + 4, pitch);
-            const uint32_t b0 = load_u32(row, off + 256, pitch), b1 = load_u32(row, off + 260, pitch);
-            uint32_t* dst = lds + (size_t)c * 4 * W4 + 4 * qd;
-            // w[j] pairs byte j of the 8-byte window at off with byte j of the window
-            // at off + 256; copy k holds w[k .. k + 3]
-            uint32_t w[7];
+    auto widen = [&]() {
 #pragma unroll
-            for (int j = 0; j < 4; j++) w[j] = pair_word(b0, a0, j);
+        for (int m = 0; m < DD_M; m++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) w[4 + j] = pair_word(b1, a1, j);
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                *(uint4*)(dst + (size_t)k * W4) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
-            c += sc;
-            qd += sq;
-            if (qd >= quads) {
-                qd -= quads;
-                c++;
+            for (int j = 0; j < 4; j++) {
+                const uint32_t a = acc16[m][j], t = lo[m][j];
+                const uint32_t x = (t & 0xffffu) + (a & 0xffffu);
+                const uint32_t y = (t >> 16) + (a >> 16);
+                lo[m][j] = (x & 0xffffu) | (y << 16);
+                cy[m] += ((x >> 16) << (4 * j)) + ((y >> 16) << (4 * j + 16));
+                acc16[m][j] = 0;
             }
+    };
+    uint32_t in_phase = 0;
+    const char* lds_lane = (const char*)lds + 16 * lane;
+    for (uint32_t i = threadIdx.x; i < nchans; i += DD_THREADS) {
+        lbase[i] = bblk[i];
+        lcinfo[i] = cinfo[i];
+    }
+    for (uint32_t i = threadIdx.x; i <= nstages; i += DD_THREADS) lsfirst[i] = sfirst[i];
+    // Wave w stages channel w of each stage.  Its raw window starts at t0 + base,
+    // read from the 16-byte aligned start s16 in 1 KiB wave loads into raw slot w;
+    // lanes past the row end read its last 16 bytes instead (padding, only under
+    // outputs at or past n).  MUBUF loads (not FLAT) keep the sums' waits partial.
+    auto issue = [&](uint32_t st) {
+        const uint32_t cc = __builtin_amdgcn_readfirstlane(lsfirst[st]);
+        const uint32_t nc = __builtin_amdgcn_readfirstlane(lsfirst[st + 1]) - cc;
+        if ((uint32_t)wave >= nc) return;
+        uint32_t ln = threadIdx.x;  // recomputed per call: hoisted per-lane values would spill
+        asm volatile("" : "+v"(ln));
+        ln &= 63;
+        const uint32_t c = cc + wave;
+        const uint32_t w4 = __builtin_amdgcn_readfirstlane(lcinfo[c]) & 0xffffu;
+        const uint32_t s16 = ((uint32_t)t0 + (uint32_t)__builtin_amdgcn_readfirstlane(lbase[c])) & ~15u;
+        const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(data + (size_t)c * pitch), 0, (int)pitch, DD_RSRC_WORD3);
+        for (uint32_t k = 0; k < w4 + 276; k += 1024) {
+            const uint32_t p = (uint32_t)min((uint64_t)s16 + k + 16u * ln, pitch - 16);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, raw + (size_t)wave * RB + k, 16, p, 0, 0, 0);
         }
-        for (uint32_t i = threadIdx.x; i < nc * DD_TRIALS; i += blockDim.x) {
-            const uint32_t c = i / DD_TRIALS;
-            const int32_t e = rblk[(size_t)cc * DD_TRIALS + i];
-            laddr[i] = (uint16_t)((((uint32_t)c * 4 + (e & 3)) * W4 + (uint32_t)(e & ~3)) * 4);
+        const __amdgpu_buffer_rsrc_t rt = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(rblk + (size_t)c * DD_TRIALS), 0, 4 * DD_TRIALS, DD_RSRC_WORD3);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rt, rrel + (size_t)wave * DD_TRIALS, 4, 4 * ln, 0, 0, 0);
+    };
+    __syncthreads();
+    issue(0);
+    for (uint32_t st = 0; st < nstages; st++) {
+        const uint32_t cc = __builtin_amdgcn_readfirstlane(lsfirst[st]);
+        const uint32_t nc = __builtin_amdgcn_readfirstlane(lsfirst[st + 1]) - cc;
+        // this wave's loads have landed; after the barrier every wave's have, and the
+        // previous stage's sums are done with the rows
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __syncthreads();
+        if ((uint32_t)wave < nc) {
+            uint32_t ln = threadIdx.x;  // as in issue
+            asm volatile("" : "+v"(ln));
+            ln &= 63;
+            const uint32_t c = cc + wave;
+            const uint32_t info = __builtin_amdgcn_readfirstlane(lcinfo[c]);
+            const uint32_t w4 = info & 0xffffu, rowoff = info >> 16, quads = w4 / 4;
+            const uint32_t mis = ((uint32_t)t0 + (uint32_t)__builtin_amdgcn_readfirstlane(lbase[c])) & 15u;
+            const uint32_t* rw0 = (const uint32_t*)(raw + (size_t)wave * RB + mis);
+            uint32_t* row = lds + rowoff / 4;
+            for (uint32_t qd = ln; qd < quads; qd += 64) {
+                const uint32_t* rw = rw0 + qd;
+                const uint32_t a0 = rw[0], a1 = rw[1], b0 = rw[64], b1 = rw[65];
+                uint32_t* dst = row + 4 * qd;
+                // w[k] pairs byte k of the 8-byte window at 4 qd with byte k of the
+                // window 256 bytes later; copy k holds w[k .. k + 3]
+                uint32_t w[7];
+#pragma unroll
+                for (int k = 0; k < 4; k++) w[k] = pair_word(b0, a0, k);
+#pragma unroll
+                for (int k = 0; k < 3; k++) w[4 + k] = pair_word(b1, a1, k);
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    *(uint4*)(dst + (size_t)k * w4) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+            }
+            const int32_t e = rrel[wave * DD_TRIALS + ln];
+            laddr[wave * DD_TRIALS + ln] = (uint16_t)(rowoff + ((uint32_t)(e & 3) * w4 + (uint32_t)(e & ~3)) * 4);
         }
         __syncthreads();
-#pragma unroll 2
-        for (uint32_t c = 0; c < nc; c++) {
-            // the wave's 8 offsets: one 16-byte broadcast read, 16 bits each
-            const uint4 A = *(const uint4*)(laddr + c * DD_TRIALS + wave * DD_M);
-            const uint32_t addr[DD_M] = {A.x & 0xffffu, A.x >> 16, A.y & 0xffffu, A.y >> 16,
-                                         A.z & 0xffffu, A.z >> 16, A.w & 0xffffu, A.w >> 16};
+        if (st + 1 < nstages) issue(st + 1);
+        // channels in pairs: one v_add3_u32 adds both channels' u16 pairs (no half
+        // carries within a phase, so the 32-bit add is two exact 16-bit adds).  The
+        // wave's 8 offsets per channel come from 16-byte broadcast reads (16 bits
+        // each), fetched one pair ahead.
+        const uint16_t* lw = laddr + wave * DD_M;
+        uint4 A = *(const uint4*)(lw), B = *(const uint4*)(lw + DD_TRIALS);
+        for (uint32_t c = 0; c + 1 < nc; c += 2) {
+            const uint32_t pa[4] = {A.x, A.y, A.z, A.w}, pb[4] = {B.x, B.y, B.z, B.w};
+            uint4 x[DD_M], y[DD_M];
 #pragma unroll
             for (int m = 0; m < DD_M; m++) {
-                const uint4 v = *(const uint4*)(lds_lane + addr[m]);
-                acc16[m][0] = pk_add(acc16[m][0], v.x);
-                acc16[m][1] = pk_add(acc16[m][1], v.y);
-                acc16[m][2] = pk_add(acc16[m][2], v.z);
-                acc16[m][3] = pk_add(acc16[m][3], v.w);
+                const uint32_t sa = m & 1 ? pa[m >> 1] >> 16 : pa[m >> 1] & 0xffffu;
+                const uint32_t sb = m & 1 ? pb[m >> 1] >> 16 : pb[m >> 1] & 0xffffu;
+                x[m] = *(const uint4*)(lds_lane + sa);
+                y[m] = *(const uint4*)(lds_lane + sb);
             }
-            if (++in_phase == phase_chans) {
+            const uint32_t cn = c + 3 < nc ? c + 2 : c;  // next pair (a repeat at the end)
+            A = *(const uint4*)(lw + cn * DD_TRIALS);
+            B = *(const uint4*)(lw + (cn + 1) * DD_TRIALS);
+#pragma unroll
+            for (int m = 0; m < DD_M; m++) {
+                acc16[m][0] = acc16[m][0] + x[m].x + y[m].x;
+                acc16[m][1] = acc16[m][1] + x[m].y + y[m].y;
+                acc16[m][2] = acc16[m][2] + x[m].z + y[m].z;
+                acc16[m][3] = acc16[m][3] + x[m].w + y[m].w;
+            }
+            // issue order: the reads of DD_AHEAD trials, then each trial's four adds
+            // followed by the two reads DD_AHEAD trials later; the next offsets go
+            // out after the last data reads
+#pragma unroll
+            for (int m = 0; m < DD_AHEAD; m++) {
+                __builtin_amdgcn_sched_group_barrier(0x0002, 2, 0);  // VALU: addresses
+                __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);  // DS read: x, y
+            }
+#pragma unroll
+            for (int m = 0; m < DD_M; m++) {
+                __builtin_amdgcn_sched_group_barrier(0x0002, 4, 0);  // VALU: adds of trial m
+                if (m + DD_AHEAD < DD_M) {
+                    __builtin_amdgcn_sched_group_barrier(0x0002, 2, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
+                }
+                if (m + DD_AHEAD == DD_M - 1) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
+            }
+            in_phase += 2;
+            if (in_phase == phase_chans) {
                 in_phase = 0;
-#pragma unroll
-                for (int m = 0; m < DD_M; m++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        acc32[m][j] += acc16[m][j] & 0xffffu;
-                        acc32[m][4 + j] += acc16[m][j] >> 16;
-                        acc16[m][j] = 0;
-                    }
+                widen();
             }
         }
+        if (nc & 1) {  // odd channel count: the last stage's final channel alone
+            const uint32_t c = nc - 1;
+            const uint4 A = *(const uint4*)(laddr + c * DD_TRIALS + wave * DD_M);
+            const uint32_t pa[4] = {A.x, A.y, A.z, A.w};
+#pragma unroll
+            for (int m = 0; m < DD_M; m++) {
+                const uint32_t sa = m & 1 ? pa[m >> 1] >> 16 : pa[m >> 1] & 0xffffu;
+                const uint4 x = *(const uint4*)(lds_lane + sa);
+                acc16[m][0] += x.x;
+                acc16[m][1] += x.y;
+                acc16[m][2] += x.z;
+                acc16[m][3] += x.w;
+            }
+            in_phase += 1;  // phase_chans is even and in_phase was even: no boundary here
+        }
     }
+    widen();
+    // totals: low 16 bits from lo, the carries above them
+    uint32_t acc32[DD_M][8];
 #pragma unroll
     for (int m = 0; m < DD_M; m++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            acc32[m][j] += acc16[m][j] & 0xffffu;
-            acc32[m][4 + j] += acc16[m][j] >> 16;
+            acc32[m][j] = (lo[m][j] & 0xffffu) + (((cy[m] >> (4 * j)) & 15u) << 16);
+            acc32[m][4 + j] = (lo[m][j] >> 16) + (((cy[m] >> (4 * j + 16)) & 15u) << 16);
         }
     const int m0i = (int)m0;
 #pragma unroll
@@ -358,38 +470,69 @@ __global__ __launch_bounds__(DD_THREADS) void k_dedisp_u8(
 
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
-    return nblk * nchans * (1 + DD_TRIALS);
+    // bases, relative delays, then the channels' row info and the stages' first channels
+    return nblk * nchans * (1 + DD_TRIALS) + 2 * (size_t)nchans + 2;
 }
 
 bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
-                      double* partial, uint32_t spread_max) {
-    // a trial reads words (e & ~3) .. (e & ~3) + 255 with e <= spread_max + 3
-    const uint64_t W4 = ((uint64_t)spread_max + 3 + 3) / 4 * 4 + DD_TILE / 2;
-    // four row copies + 16-bit offsets per channel; the row copies must stay below
-    // 64 KiB (16-bit offsets), which also lets two 512-thread workgroups share a CU
-    const uint64_t per_chan = 16 * W4 + 2 * DD_TRIALS;
-    const uint64_t budget = 64 * 1024;
-    if (per_chan > budget) return false;
-    uint32_t C = (uint32_t)std::min<uint64_t>(budget / per_chan, 64);
-    C = std::min(C, nchans);
+                      double* partial, const uint32_t* spread) {
+    // 4-bit carry counters per output: every total must stay below 2^20;
+    // 16-byte aligned raw loads inside each row; 32-bit buffer offsets
+    if (nchans == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
+    if (pitch % 16 || pitch < 16 || pitch >= (1ull << 31)) return false;
+    // Row length of channel c: a trial reads words (e & ~3) .. (e & ~3) + 255 with
+    // e <= spread[c] + 3.  Its raw window (read from a 16-byte aligned start) takes
+    // W4 + 276 bytes, in whole 1 KiB wave loads; RB is the raw slot size.
+    std::vector<uint32_t> w4(nchans), info(nchans), first;
+    uint64_t RB = 1024;
+    for (uint32_t c = 0; c < nchans; c++) {
+        const uint64_t w = ((uint64_t)spread[c] + 3 + 3) / 4 * 4 + DD_TILE / 2;
+        if (w > 0xffff) return false;
+        w4[c] = (uint32_t)w;
+        RB = std::max<uint64_t>(RB, (w + 276 + 1023) / 1024 * 1024);
+    }
+    if (RB * 2 > DD_RAW_BYTES) return false;
+    const uint32_t slots = std::min<uint32_t>(DD_RAW_CHANS, (uint32_t)(DD_RAW_BYTES / RB));
+    // stages: consecutive channels whose four row copies fit DD_ROW_BYTES (< 64 KiB,
+    // for the 16-bit offsets), one per wave; an even count (channels are added in
+    // pairs) except in the last stage
+    for (uint32_t c = 0; c < nchans;) {
+        uint32_t k = 0, rows = 0;
+        while (c + k < nchans && k < slots && rows + 16 * w4[c + k] <= (uint32_t)DD_ROW_BYTES)
+            rows += 16 * w4[c + k++];
+        if (c + k < nchans) k &= ~1u;
+        if (k == 0) return false;
+        first.push_back(c);
+        for (uint32_t j = 0, off = 0; j < k; off += 16 * w4[c + j], j++) info[c + j] = w4[c + j] | off << 16;
+        c += k;
+    }
+    const uint32_t nstages = (uint32_t)first.size();
+    first.push_back(nchans);
+    // dynamic LDS: bases and row info per channel, first channel per stage
+    const size_t shm = 8 * (size_t)nchans + 4 * (size_t)(nstages + 1);
+    if (shm > 80 * 1024) return false;
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
     int32_t* bases = table;
     int32_t* rel = table + nblk * nchans;
+    uint32_t* cinfo = (uint32_t*)(rel + nblk * nchans * DD_TRIALS);
+    uint32_t* sfirst = cinfo + nchans;
+    if (hipMemcpyAsync(cinfo, info.data(), (size_t)nchans * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(sfirst, first.data(), first.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+        return false;
     hipLaunchKernelGGL(k_dd_table, dim3((unsigned)((nblk * nchans + 255) / 256)), dim3(256), 0, s,
                        delays, nchans, ntrials, bases, rel);
+    // channels per phase: no 16-bit half can overflow; even, so pairs never straddle
     uint32_t phase = 65535u / (maxval ? maxval : 1);
-    phase = phase > 4096 ? 4096 : phase;
-    const size_t shm = (size_t)C * per_chan;
-    if (shm > 64 * 1024)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dedisp_u8),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    phase = (phase > 4096 ? 4096 : phase) & ~1u;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dedisp_u8),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     const uint32_t ntiles = (n + DD_TILE - 1) / DD_TILE;
     const uint64_t total = (uint64_t)ntiles * nblk;
     hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(DD_THREADS), shm, s, data, pitch, nchans,
                        phase, bases, rel, ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16,
-                       m0, inv_s0, partial, (uint32_t)W4, C);
+                       m0, inv_s0, partial, cinfo, sfirst, nstages, (uint32_t)RB);
     return true;
 }
 

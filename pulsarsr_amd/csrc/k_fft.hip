@@ -396,8 +396,9 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ s
     float2* buf = smem;  // [M2][C], element r of column col at r*C + col
     float2* T1 = smem + C * M2;
     float2* T4 = T1 + NT;
-    const uint32_t trial = blockIdx.y;
-    const uint32_t c0 = blockIdx.x * C;
+    const uint2 wb = xcd_block();  // column groups sharing cache lines on one XCD
+    const uint32_t trial = wb.y;
+    const uint32_t c0 = wb.x * C;
     const size_t row = (size_t)trial * series_pitch;
     const float mean = F16 ? 0.f : mean32[trial];
     const float corr = F16 ? corr16[trial] : 0.f;

@@ -100,13 +100,14 @@ __global__ __launch_bounds__(256) void k_sort_blocks(
     extern __shared__ __align__(16) uint32_t tile[];
     constexpr int E = F16 ? 8 : 4;                            // keys per 16-byte vector
     constexpr int VPT = (int)(SB_BLOCKS * 64 / E / 256);      // vectors per thread
-    const uint32_t trial = blockIdx.y;
-    const uint32_t k1a = blockIdx.x * cw;
+    const uint2 wb = xcd_block();  // column groups sharing cache lines on one XCD
+    const uint32_t trial = wb.y;
+    const uint32_t k1a = wb.x * cw;
     const uint32_t k_lo = k1a * m2;
     const uint32_t span = cw * m2;             // natural bins of the columns
     const uint32_t nbk = (span + 63) / 64;     // tile blocks
     const size_t srow = (size_t)trial * spec_pitch;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (wb.x == 0 && threadIdx.x == 0) {
         double S = 0.0;
         for (uint32_t i = 0; i < npart; i++) S += partial[(size_t)trial * npart + i];
         floors[trial] = (float)(S / (double)nbins * 1e-6);
@@ -205,8 +206,9 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
     constexpr int E = F16 ? 8 : 4;                                  // keys per 16-byte load
     constexpr int NV = (int)((MB_BLOCKS + 4) * 64 / E);             // vectors staged
     constexpr int VPT = (NV + (int)MB_BLOCKS - 1) / (int)MB_BLOCKS;  // per thread
-    const uint32_t trial = blockIdx.y;
-    const uint32_t b0 = blockIdx.x * MB_BLOCKS;
+    const uint2 wb = xcd_block();
+    const uint32_t trial = wb.y;
+    const uint32_t b0 = wb.x * MB_BLOCKS;
     const size_t srow = (size_t)trial * spec_pitch;
     const uint32_t nsorted = (m + 63) / 64;
     uint4 q[VPT];
@@ -406,11 +408,12 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
     __shared__ uint32_t lbin[HS_T];  // their thread index
     __shared__ float hv[HS_T][HS_NJ + 2];  // odd row stride: conflict-free column reads
     __shared__ uint32_t nlist;
-    const uint32_t trial = blockIdx.y;
+    const uint2 wb = xcd_block();  // neighbouring bins' harmonics share lines on one XCD
+    const uint32_t trial = wb.y;
     const size_t row = (size_t)trial * spec_pitch;
     const float* scl = scales + (size_t)trial * nblocks;
     const int tid = threadIdx.x;
-    const int64_t k = (int64_t)a.p.first - 1 + (int64_t)blockIdx.x * HS_OUT + tid;
+    const int64_t k = (int64_t)a.p.first - 1 + (int64_t)wb.x * HS_OUT + tid;
     const double NEG_INF = -__builtin_inf();
     double offset = 0.0;
 #pragma unroll
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
     __syncthreads();
     if ((uint32_t)tid < G) {
         const uint32_t t = lbin[tid];
-        const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)blockIdx.x * HS_OUT + t;
+        const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)wb.x * HS_OUT + t;
         double S = lsum[tid];
         uint32_t j = 2;
         for (uint32_t l = 1; l < a.p.nlevels; l++) {

@@ -52,6 +52,29 @@ def test_dedisperse_bit_exact(ctx, nbits):
         assert (got[i].view(np.uint32) == ref.view(np.uint32)).all(), "DM %g" % dm
 
 
+@pytest.mark.parametrize("nchans,lo,hi", [(601, 200, 256), (1023, 240, 256), (4200, 250, 256)])
+def test_dedisperse_many_channels_exact(ctx, nchans, lo, hi):
+    """Channel counts past one 16-bit phase (256 channels of 8-bit samples), odd
+    counts, totals whose low halves wrap several times, and (4200 x 255 >= 2^20)
+    the channel-ordered f32 kernel taking over: all bit-exact."""
+    rng = np.random.default_rng(nchans)
+    ns = 1 << 13
+    payload = rng.integers(lo, hi, (ns, nchans), dtype=np.uint8).tobytes()
+    foff = -300.0 / nchans
+    obs = dedisperse.Observation.from_packed(payload, 8, nchans, 64e-6, 1500.0, foff)
+    data, ns2 = oracle.decode(payload, 8, nchans)
+    freqs = 1500.0 + foff * np.arange(nchans)
+    dms = np.array([0.0, 3.0, 41.0, 97.5, 150.0])
+    n = ns - int(oracle.delays(freqs, 64e-6, 150.0).max())
+    print("dedisperse %d channels, samples %d..%d, %d outputs" % (nchans, lo, hi - 1, n), flush=True)
+    got = dedisperse.apply_dms(obs, dms, n)
+    for i, dm in enumerate(dms):
+        ref = oracle.apply(data, ns2, oracle.delays(freqs, 64e-6, dm), n)
+        assert ref.max() > 65535.0
+        assert (got[i].view(np.uint32) == ref.view(np.uint32)).all(), "DM %g" % dm
+    obs.close()
+
+
 def test_dedisperse_full_length_and_unsorted_dms(ctx):
     w = small_workload(8)
     _, obs, data, ns = setup(w)
@@ -166,6 +189,35 @@ def test_off_bin_pulse_train_retains_fundamental(ctx):
     assert abs(kept[0].frequency_hz - 1.0 / period) < 0.2
     ref = oracle.periodicity(x, dt, 18.0)
     assert match_lists([c], [ref], 6.0) < SCORE_RTOL
+
+
+@pytest.mark.parametrize("precision", [PRECISION_FP32, PRECISION_FP16])
+def test_batched_search_matches_one_batch(precision):
+    """Trial batches overlapped on two streams (and run serially) give the
+    candidates of a single batch, scattered back to the callers' unsorted
+    trial order."""
+    from pulsarsr_amd import Context
+    w = small_workload(8, nsamples=1 << 16)
+    payload = synth.generate(w)
+    dms = np.random.default_rng(9).permutation(np.arange(200) * 0.4)
+    d = oracle.delays(w.frequencies(), w.tsamp, float(dms.max()))
+    n = (w.nsamples - 2 * int(d.max())) // 2
+    cfg = search.Config(min_hz=w.min_hz, threshold=w.threshold, precision=precision)
+    res = {}
+    for name, overlap, cap in (("one", False, 0), ("overlapped", True, 120 << 20), ("serial", False, 120 << 20)):
+        c = Context(0, overlap=overlap)
+        obs = dedisperse.Observation.from_packed(payload, w.nbits, w.nchans, w.tsamp, w.fch1, w.foff, ctx=c)
+        c.set_memory_cap(cap)
+        out, counts, _ = search.search_raw(obs, dms, n, cfg)
+        st = c.stage_times()
+        print("%s: %d batches, %.2f ms" % (name, st["launches"], st["total_ms"]), flush=True)
+        res[name] = (bytes(out), counts.copy(), st["launches"])
+        obs.close()
+        c.close()
+    assert res["one"][2] == 1 and res["overlapped"][2] >= 2 and res["serial"][2] >= 2
+    assert res["one"][1].sum() > 0
+    for k in ("overlapped", "serial"):
+        assert (res[k][1] == res["one"][1]).all() and res[k][0] == res["one"][0], k
 
 
 def test_search_is_deterministic(ctx):
