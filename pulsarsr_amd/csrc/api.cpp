@@ -712,10 +712,11 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return dms[a] < dms[b]; });
     std::vector<double> sorted(ntrials);
     for (size_t i = 0; i < ntrials; i++) sorted[i] = dms[order[i]];
-    // trials per chunk: keep the working set (two series slots, one set of
+    // trials per chunk: keep the working set (the series slots, one set of
     // spectral buffers) near 16 GiB
+    const bool serial = (c->flags & PSR_CTX_OVERLAP) == 0;
     const size_t es = p.fp16 ? 2 : 4;
-    const size_t per_trial = 2 * (size_t)g.series_pitch * es +
+    const size_t per_trial = (serial ? 1 : 2) * (size_t)g.series_pitch * es +
                              3 * (size_t)g.spec_pitch * es + (size_t)PEAK_CAP * sizeof(RawPeak) +
                              cap_per_trial * sizeof(CandidateOut) + (size_t)o->nchans * 4 + 4096;
     // (half the ctx memory cap when one is set); whole 64-trial dedispersion blocks
@@ -725,7 +726,6 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
     if (chunk >= DEDISP_BLOCK_TRIALS) chunk -= chunk % DEDISP_BLOCK_TRIALS;
     chunk = std::min(chunk, ntrials);
     const size_t nchunks = (ntrials + chunk - 1) / chunk;
-    const bool serial = (c->flags & PSR_CTX_OVERLAP) == 0;
     hipStream_t sa = c->stream, sb = serial ? c->stream : c->stream2;
     // both slots and the spectral buffers are sized for the first (largest)
     // chunk before anything is queued, so no allocation happens while kernels run

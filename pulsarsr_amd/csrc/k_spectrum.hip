@@ -14,6 +14,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace psr {
@@ -202,7 +204,11 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
     const void* __restrict__ sorted, const void* __restrict__ spec_t, uint32_t m, uint32_t nbins,
     uint32_t nblocks, uint64_t spec_pitch, const float* __restrict__ floors,
     float* __restrict__ scales) {
-    __shared__ uint32_t s[(MB_BLOCKS + 4) * MB_STRIDE];
+    // FP16 keys are staged as 16-bit values (66 per block: an odd word stride),
+    // which halves the footprint and doubles the workgroups per CU
+    using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
+    constexpr int STRIDE = F16 ? 2 * 33 : (int)MB_STRIDE;
+    __shared__ KeyT s[(MB_BLOCKS + 4) * STRIDE];
     constexpr int E = F16 ? 8 : 4;                                  // keys per 16-byte load
     constexpr int NV = (int)((MB_BLOCKS + 4) * 64 / E);             // vectors staged
     constexpr int VPT = (NV + (int)MB_BLOCKS - 1) / (int)MB_BLOCKS;  // per thread
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
             // block `nsorted` (m a multiple of 64) holds Nyquist alone
             if (gb == (int64_t)nsorted && gb < (int64_t)nblocks && i0 + e == 0)
                 key = load_key(spec_t, F16, srow + m);
-            s[lb * MB_STRIDE + i0 + e] = key;
+            s[lb * STRIDE + i0 + e] = (KeyT)key;  // FP16: KEY_INF keeps its low 16 bits
         }
     }
     __syncthreads();
@@ -246,7 +252,7 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
     int nn[5];
     uint32_t L[5], U[5];
     int p[5];
-    const uint32_t* A[5];
+    const KeyT* A[5];
     int len = 0;
 #pragma unroll
     for (int j = 0; j < 5; j++) {
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
             cnt = s1 > s0 ? (int)(s1 - s0) : 0;
         }
         nn[j] = cnt;
-        A[j] = s + (threadIdx.x + j) * MB_STRIDE;
+        A[j] = s + (threadIdx.x + j) * STRIDE;
         p[j] = cnt >> 1;
         len += cnt;
     }
@@ -389,13 +395,16 @@ __device__ __forceinline__ double interp_offset(double a, double b, double c) {
 constexpr int HS_T = 256;       // threads: scores for HS_T bins
 constexpr int HS_OUT = HS_T - 2;  // peak tests per workgroup
 constexpr int HS_NJ = 15;         // harmonics j = 2 .. 16
+constexpr int HS_LIST = 64;       // listed bins summed per round
 
 // One workgroup scores HS_T consecutive bins.  Level 0 (j = 1) is computed for
 // every bin; higher levels only for bins whose whitened power reaches 4
 // (lib.rs:97-99), typically a few percent.  Those bins are compacted into a
-// list, their (bin, harmonic) loads are spread over all threads (one round of
-// memory latency instead of fifteen), and one thread per listed bin then adds
-// its harmonics in order j = 2, 3, ... in f64, as the reference does.
+// list; HS_LIST of them at a time, their (bin, harmonic) loads are spread over
+// all threads (one round of memory latency instead of fifteen), and one thread
+// per listed bin then adds its harmonics in order j = 2, 3, ... in f64, as the
+// reference does.  The small list keeps the LDS footprint low enough for eight
+// workgroups per CU.
 __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, int fp16,
                                                  uint64_t spec_pitch, uint32_t nbins,
                                                  uint32_t nblocks,
@@ -405,8 +414,8 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
     __shared__ double sc[5][HS_T];
     __shared__ double lpos[HS_T];   // k + offset of listed bins
     __shared__ double lsum[HS_T];   // their j = 1 term
-    __shared__ uint32_t lbin[HS_T];  // their thread index
-    __shared__ float hv[HS_T][HS_NJ + 2];  // odd row stride: conflict-free column reads
+    __shared__ uint8_t lbin[HS_T];  // their thread index
+    __shared__ float hv[HS_LIST][HS_NJ + 2];  // odd row stride: conflict-free column reads
     __shared__ uint32_t nlist;
     const uint2 wb = xcd_block();  // neighbouring bins' harmonics share lines on one XCD
     const uint32_t trial = wb.y;
@@ -442,32 +451,35 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
         const uint32_t slot = atomicAdd(&nlist, 1u);
         lpos[slot] = (double)k + offset;
         lsum[slot] = S1;
-        lbin[slot] = (uint32_t)tid;
+        lbin[slot] = (uint8_t)tid;
     }
     __syncthreads();
     const uint32_t G = nlist;
-    for (uint32_t it = tid; it < G * HS_NJ; it += HS_T) {
-        const uint32_t g = it / HS_NJ, jj = it - g * HS_NJ;
-        const double position = lpos[g] * (double)(jj + 2);
-        uint64_t index = (uint64_t)round(position);
-        if (index > nbins - 1) index = nbins - 1;
-        const uint32_t iu = (uint32_t)index;
-        hv[g][jj] = whiten_val(raw_at(raw, fp16, row + iu), scl[iu >> 6], iu, a);
-    }
-    __syncthreads();
-    if ((uint32_t)tid < G) {
-        const uint32_t t = lbin[tid];
-        const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)wb.x * HS_OUT + t;
-        double S = lsum[tid];
-        uint32_t j = 2;
-        for (uint32_t l = 1; l < a.p.nlevels; l++) {
-            const uint32_t h = 1u << l;
-            if (kk > (int64_t)a.p.end_h[l]) break;
-            for (; j <= h; j++) S += (double)hv[tid][j - 2];
-            sc[l][t] = (S - (double)h) / a.sqrt_h[l];
+    for (uint32_t g0 = 0; g0 < G; g0 += HS_LIST) {
+        const uint32_t ng = min((uint32_t)HS_LIST, G - g0);
+        for (uint32_t it = tid; it < ng * HS_NJ; it += HS_T) {
+            const uint32_t g = it / HS_NJ, jj = it - g * HS_NJ;
+            const double position = lpos[g0 + g] * (double)(jj + 2);
+            uint64_t index = (uint64_t)round(position);
+            if (index > nbins - 1) index = nbins - 1;
+            const uint32_t iu = (uint32_t)index;
+            hv[g][jj] = whiten_val(raw_at(raw, fp16, row + iu), scl[iu >> 6], iu, a);
         }
+        __syncthreads();
+        if ((uint32_t)tid < ng) {
+            const uint32_t t = lbin[g0 + tid];
+            const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)wb.x * HS_OUT + t;
+            double S = lsum[g0 + tid];
+            uint32_t j = 2;
+            for (uint32_t l = 1; l < a.p.nlevels; l++) {
+                const uint32_t h = 1u << l;
+                if (kk > (int64_t)a.p.end_h[l]) break;
+                for (; j <= h; j++) S += (double)hv[tid][j - 2];
+                sc[l][t] = (S - (double)h) / a.sqrt_h[l];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (tid < 1 || tid > HS_OUT) return;
     if (k < (int64_t)a.p.first + 1) return;
     for (uint32_t l = 0; l < a.p.nlevels; l++) {
