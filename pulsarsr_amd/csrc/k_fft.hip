@@ -23,6 +23,11 @@
 
 #include "common.hpp"
 
+// The transforms may fuse multiply-adds (the library is otherwise built without
+// contraction, for the order-exact f32 sums elsewhere): FFT parity is normwise,
+// and a fused multiply-add only drops a rounding.
+#pragma clang fp contract(fast)
+
 namespace psr {
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
