@@ -72,14 +72,14 @@ def top_families(cands, n, tsamp, k=20):
 
 # HBM bytes per unit of each kernel on the FP16 path (FETCH_SIZE + WRITE_SIZE,
 # calibrated against a 1 GiB device copy), from the rocprofv3 passes recorded in
-# profiles/hbm_traffic_headline-1024ch-8bit.md (960-trial launches).
+# profiles/hbm_traffic_headline-1024ch-8bit.md (one 768-trial launch).
 MEASURED_TRAFFIC_BYTES_PER_UNIT = {
-    "k_dedisp_u8": 14.54,
-    "k_fft_pass1": 12.67,
-    "k_fft_pass2": 11.51,
-    "k_sort_blocks": 8.08,
-    "k_block_median": 2.15,
-    "k_harmsum": 4.22,
+    "k_dedisp_u8": 8.60,
+    "k_fft_pass1": 10.57,
+    "k_fft_pass2": 11.43,
+    "k_sort_blocks": 6.02,
+    "k_block_median": 2.07,
+    "k_harmsum": 1.82,
 }
 
 
