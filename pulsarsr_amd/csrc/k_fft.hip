@@ -418,13 +418,10 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ s
     build_pass_tw<M2>(T1, T4, cos_tables);
 #pragma unroll
     for (int i = 0; i < PER; i++) buf[threadIdx.x + i * FFT_THREADS] = v[i];
-    __syncthreads();
-    fft_passes<M2, C, false, 0, 1>(buf, T1, T4);
-    float2* w = work + (size_t)trial * M;
+    // the output twiddles, loaded while the transform runs: thread's column is
+    // fixed (FFT_THREADS is a multiple of C); k2 steps by FFT_THREADS / C
     const float2* tlo = tw4;
     const float2* thi = tw4 + 2048;
-    // thread's column is fixed (FFT_THREADS is a multiple of C); k2 steps by
-    // FFT_THREADS / C
     const int col = threadIdx.x % C, k20 = threadIdx.x / C;
     constexpr int KS = FFT_THREADS / C;
     float2 tl[PER], th[PER];
@@ -434,6 +431,9 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ s
         tl[i] = tlo[a & 2047];
         th[i] = thi[a >> 11];
     }
+    __syncthreads();
+    fft_passes<M2, C, false, 0, 1>(buf, T1, T4);
+    float2* w = work + (size_t)trial * M;
 #pragma unroll
     for (int i = 0; i < PER; i++) {
         const int k2 = k20 + KS * i;
