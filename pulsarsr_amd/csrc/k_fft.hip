@@ -12,12 +12,16 @@
 //           workgroup) and the power, written as spec_t[k2*m1 + k1].
 // The transposed spectrum is put back in natural order by the whitening kernel.
 //
-// The in-LDS transforms are Stockham passes of radix 8 or 16 (2 or 4 for the
-// remainder): each thread loads R points, applies the pass twiddles, runs a
-// register DFT_R and stores the outputs in autosorted order, so a 4096-point
-// transform takes three LDS round trips.  Butterflies are FP32; every twiddle
-// comes from FP64-generated tables (LDS-resident per workgroup).  The
-// upper half of the zero-padded input is never read from memory.
+// The common shapes (m2 = 512 in pass 1, m1 = 4096 in pass 2) run
+// register-first: each thread loads the 16 points of one radix-16 butterfly
+// straight from memory, so the first stage needs no LDS, and the remaining
+// stages are regrouped by one (pass 1) or two (pass 2) LDS exchanges plus a
+// lane swizzle, with one barrier per exchange.  Other sizes use Stockham
+// passes of radix 8 or 16 (2 or 4 for the remainder): each thread loads R
+// points, applies the pass twiddles, runs a register DFT_R and stores the
+// outputs in autosorted order.  Butterflies are FP32; every twiddle comes from
+// FP64-generated tables or exact constants.  The upper half of the
+// zero-padded input is never read from memory.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -385,7 +389,95 @@ __global__ __launch_bounds__(512, 4) void k_fft_single(const void* __restrict__ 
     if (threadIdx.x == 0) partial[trial] = s;
 }
 
-// --------------------------------------------------------------- pass 1
+// ------------------------------------------------- pass 1, register-first
+// Length-512 column FFTs, 16 columns per workgroup.  With row r = j + 32 m
+// and output k2 = k1 + 16 q:
+//   X[k1 + 16 q] = sum_j W_32^(j q) W_512^(j k1) sum_m x[j + 32 m] W_16^(m k1).
+// Thread (j, col) loads rows j + 32 m (m < 16) of column col and runs the
+// inner DFT_16 on its registers.  One LDS exchange gives thread (k1, col, j0)
+// the values j = j0 + 2 j1 (j1 < 16) for a DFT_16 over j1; the final radix 2
+// over j0 pairs lanes l and l ^ 16 (j0 is lane bit 4) through a swizzle.
+constexpr float C32A = 0.98078528040323044913f;  // cos(pi/16)
+constexpr float S32A = 0.19509032201612826785f;  // sin(pi/16)
+constexpr float C32B = 0.83146961230254523708f;  // cos(3pi/16)
+constexpr float S32B = 0.55557023301960222474f;  // sin(3pi/16)
+__device__ __forceinline__ float2 w32(int q) {  // W_32^q, q < 16
+    switch (q) {
+        case 0: return make_float2(1.f, 0.f);
+        case 1: return make_float2(C32A, -S32A);
+        case 2: return make_float2(C16, -S16);
+        case 3: return make_float2(C32B, -S32B);
+        case 4: return make_float2(RS2, -RS2);
+        case 5: return make_float2(S32B, -C32B);
+        case 6: return make_float2(S16, -C16);
+        case 7: return make_float2(S32A, -C32A);
+        case 8: return make_float2(0.f, -1.f);
+        case 9: return make_float2(-S32A, -C32A);
+        case 10: return make_float2(-S16, -C16);
+        case 11: return make_float2(-S32B, -C32B);
+        case 12: return make_float2(-RS2, -RS2);
+        case 13: return make_float2(-C32B, -S32B);
+        case 14: return make_float2(-C16, -S16);
+        default: return make_float2(-C32A, -S32A);
+    }
+}
+__device__ __forceinline__ float swz16(float x) {  // value of lane l ^ 16 (within 32-lane halves)
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x401f));
+}
+template <bool F16>
+__global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ series,
+                                                    const float* __restrict__ mean32,
+                                                    const float* __restrict__ corr16, uint32_t n,
+                                                    uint32_t series_pitch, uint32_t M, uint32_t m1,
+                                                    float2* __restrict__ work,
+                                                    const float2* __restrict__ tw4,
+                                                    const float* __restrict__ cos_tables) {
+    constexpr int C = 16, JP = 272;  // exchange rows of 256 + 16 (bank offset between rows)
+    __shared__ float2 buf[32 * JP];
+    __shared__ float2 P[256];  // W_M^(16 (c0 + col) q1) at q1 * 16 + col
+    const uint2 wb = xcd_block();
+    const uint32_t trial = wb.y, c0 = wb.x * C;
+    const size_t row = (size_t)trial * series_pitch;
+    const float mean = F16 ? 0.f : mean32[trial];
+    const float corr = F16 ? corr16[trial] : 0.f;
+    const int t = threadIdx.x, col = t & 15, j = t >> 4;
+    float2 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+        v[m] = load_pair<F16>(series, row, 2u * (c0 + col + m1 * (uint32_t)(j + 32 * m)), n, mean, corr);
+    const float* C512 = cos_tables + costab_offset(512);
+    const float2 w1 = twiddle<512>(C512, j), w4 = twiddle<512>(C512, 4 * j);
+    const float2* tlo = tw4;
+    const float2* thi = tw4 + 2048;
+    const int j0 = (t >> 4) & 1, k1 = t >> 5;
+    const uint32_t c = c0 + col;
+    const uint32_t ab = (uint32_t)(((uint64_t)c * (uint32_t)(k1 + 256 * j0)) & (M - 1));
+    const float2 base = cmul(thi[ab >> 11], tlo[ab & 2047]);
+    if (t < 256) {
+        const uint32_t ap = (uint32_t)(((uint64_t)(c0 + (t & 15)) * (16u * (uint32_t)(t >> 4))) & (M - 1));
+        P[t] = cmul(thi[ap >> 11], tlo[ap & 2047]);
+    }
+    dft<16>(v);
+    apply_tw<16>(v, w1, w4);
+#pragma unroll
+    for (int k = 0; k < 16; k++) buf[j * JP + k * 16 + col] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int j1 = 0; j1 < 16; j1++) v[j1] = buf[(j0 + 2 * j1) * JP + k1 * 16 + col];
+    dft<16>(v);
+    const float sgn = j0 ? -1.f : 1.f;
+    float2* w = work + (size_t)trial * M;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const float2 u = q == 0 ? v[q] : cmul(v[q], j0 ? w32(q) : make_float2(1.f, 0.f));
+        const float2 p = make_float2(swz16(u.x), swz16(u.y));
+        const float2 o = make_float2(p.x + sgn * u.x, p.y + sgn * u.y);
+        const uint32_t k2 = (uint32_t)(k1 + 16 * q + 256 * j0);
+        w[(size_t)k2 * m1 + c] = cmul(o, cmul(base, P[q * 16 + col]));
+    }
+}
+
+// ------------------------------------------- pass 1, general (m2 = 1024)
 template <int M2, bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ series,
                                                    const float* __restrict__ mean32,
@@ -517,6 +609,101 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__
     if (threadIdx.x == 0) partial[(size_t)trial * (m2 / 2 + 1) + r1] = s;
 }
 
+// ------------------------------------------------- pass 2, register-first
+// Rows r1 and r2 = m2 - r1 (length 4096) as 16 x 16 x 16: thread (row, tt)
+// loads elements tt + 256 m (m < 16) and runs the first radix-16 stage on
+// them; two LDS exchanges regroup for the other two stages.  The last stage
+// places the r2c partner of every output (row r2, column 4095 - k) in lane
+// l ^ 16, which a swizzle reaches.  Row 0 (its own partner at 4096 - k)
+// takes one more LDS exchange.
+template <bool F16>
+__global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict__ work, uint32_t n,
+                                                    uint32_t M, uint32_t m2,
+                                                    void* __restrict__ spec, uint64_t spec_pitch,
+                                                    double* __restrict__ partial,
+                                                    const float2* __restrict__ twrow,
+                                                    const float* __restrict__ cos_tables) {
+    constexpr int M1 = 4096, JP = 272, RS = 16 * JP;
+    __shared__ float2 buf[2 * RS];
+    __shared__ double red[FFT_THREADS / 64];
+    const uint32_t trial = blockIdx.y;
+    const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
+    const float2* w = work + (size_t)trial * M;
+    const int t = threadIdx.x;
+    float2 v[16];
+    {  // stage A: radix 16 over m, twiddle W_4096^(tt k1)
+        const int row = t >> 8, tt = t & 255;
+        const float2* src = w + (size_t)(row ? r2 : r1) * M1 + tt;
+#pragma unroll
+        for (int m = 0; m < 16; m++) v[m] = src[256 * m];
+        const float* C4096 = cos_tables + costab_offset(4096);
+        const float2 w1 = twiddle<4096>(C4096, tt), w4 = twiddle<4096>(C4096, 4 * tt);
+        dft<16>(v);
+        apply_tw<16>(v, w1, w4);
+#pragma unroll
+        for (int k = 0; k < 16; k++) buf[row * RS + k * JP + (tt >> 4) * 16 + (tt & 15)] = v[k];
+    }
+    __syncthreads();
+    {  // stage B1: radix 16 over b (tt = a + 16 b), twiddle W_256^(a q1)
+        const int row = t >> 8, a = t & 15, k1 = (t >> 4) & 15;
+#pragma unroll
+        for (int b = 0; b < 16; b++) v[b] = buf[row * RS + k1 * JP + b * 16 + a];
+        const float* C256 = cos_tables + costab_offset(256);
+        const float2 w1 = twiddle<256>(C256, a), w4 = twiddle<256>(C256, 4 * a);
+        dft<16>(v);
+        apply_tw<16>(v, w1, w4);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) buf[row * RS + q * JP + k1 * 16 + (a ^ k1)] = v[q];
+    }
+    __syncthreads();
+    // stage B2: radix 16 over a; lane l holds row (l >> 4) & 1 and, for row 1,
+    // the mirrored (k1, q1) of lane l ^ 16's row-0 pair
+    const int l = t & 63, row = (l >> 4) & 1, i16 = l & 15, qq = 2 * (t >> 6) + (l >> 5);
+    const int k1 = row ? 15 - i16 : i16, q1 = row ? 15 - qq : qq;
+#pragma unroll
+    for (int a = 0; a < 16; a++) v[a] = buf[row * RS + q1 * JP + k1 * 16 + (a ^ k1)];
+    dft<16>(v);  // v[q2] = X_row[k1 + 16 q1 + 256 q2]
+    const float inv_n = 1.0f / (float)n;
+    const size_t srow = (size_t)trial * spec_pitch;
+    const uint32_t rowk = row ? r2 : r1;
+    const float* C8192 = cos_tables + costab_offset(8192);
+    const float2 base = cmul(twrow[rowk], twiddle<8192>(C8192, k1 + 16 * q1));
+    double acc = 0.0;
+    if (r1 == 0) {  // row 0 twice: partner of k is 4096 - k in the same row
+        __syncthreads();
+        if (row == 0)
+#pragma unroll
+            for (int q = 0; q < 16; q++) buf[k1 + 16 * q1 + 256 * q] = v[q];
+        __syncthreads();
+        if (row == 0) {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int k = k1 + 16 * q1 + 256 * q;
+                const float2 zk = v[q], zp = buf[(M1 - k) & (M1 - 1)];
+                const float p = r2c_power(zk, zp, cmul(base, w32(q)));
+                acc += store_power<F16>(spec, srow + k, p, inv_n);
+                if (k == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
+                    const float xm = zk.x - zk.y;
+                    acc += store_power<F16>(spec, srow + M, xm * xm, inv_n);
+                }
+            }
+        }
+    } else {
+        const bool keep = row == 0 || r1 != r2;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const float2 s = v[15 - q];
+            const float2 zp = make_float2(swz16(s.x), swz16(s.y));
+            const float p = r2c_power(v[q], zp, cmul(base, w32(q)));
+            const int k = k1 + 16 * q1 + 256 * q;
+            if (keep) acc += store_power<F16>(spec, srow + (size_t)rowk * M1 + k, p, inv_n);
+        }
+    }
+    const double sum = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[(size_t)trial * (m2 / 2 + 1) + r1] = sum;
+}
+
 // ------------------------------------------------------------- launchers
 template <typename K>
 static void allow_lds(K kernel, size_t bytes) {
@@ -538,6 +725,11 @@ static void launch_single(hipStream_t s, const SpecGeom& g, const void* series,
 template <int M1, bool F16>
 static void launch_pass2(hipStream_t s, const SpecGeom& g, uint32_t ntrials, const float2* work,
                          void* spec, double* partial, const float2* twrow, const float* cos_tables) {
+    if constexpr (M1 == 4096) {
+        hipLaunchKernelGGL((k_fft_pass2r<F16>), dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), 0,
+                           s, work, g.n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
+        return;
+    }
     const size_t shm =
         (size_t)(2 * M1 + ((2 * M1) >> 4) + 1 + 2 * (tw_total(M1) + 1)) * sizeof(float2);
     allow_lds(k_fft_pass2<M1, F16>, shm);
@@ -549,6 +741,12 @@ template <int M2, bool F16>
 static void launch_pass1(hipStream_t s, const SpecGeom& g, const void* series, const float* mean32,
                          const float* corr16, uint32_t ntrials, float2* work, const float2* tw4,
                          const float* cos_tables) {
+    if constexpr (M2 == 512) {  // m1 = M / 512 >= 16 columns
+        hipLaunchKernelGGL((k_fft_pass1r<F16>), dim3(g.m1 / 16, ntrials), dim3(FFT_THREADS), 0, s,
+                           series, mean32, corr16, g.n, g.series_pitch, g.m, g.m1, work, tw4,
+                           cos_tables);
+        return;
+    }
     constexpr int C = 8192 / M2;
     const size_t shm = (size_t)(C * M2 + 2 * (tw_total(M2) + 1)) * sizeof(float2);
     allow_lds(k_fft_pass1<M2, F16>, shm);

@@ -113,7 +113,7 @@ def test_f32_channel_order_sums(ctx):
         assert (got[i].view(np.uint32) == ref.view(np.uint32)).all()
 
 
-@pytest.mark.parametrize("n", [1000, 4096, 8000, 32583, 100000, 263500])
+@pytest.mark.parametrize("n", [1000, 4096, 8000, 32583, 100000, 263500, 1500000, 2097152, 2300000])
 def test_spectrum_matches_oracle(ctx, n):
     rng = np.random.default_rng(n)
     t = np.arange(n)
