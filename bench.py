@@ -23,12 +23,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# MI355X: 256 CUs at 2.4 GHz.  LDS: 256 B per CU per clock of ds_read_b128 (a
-# probe limited by its own VALU reached 215 B/clk/CU).  VALU: 64 lanes per CU per
-# clock; v_add3_u32 on u16 pairs (the dedispersion add) does four 16-bit adds per
-# lane.  Each dedispersion add reads a 2-byte operand from LDS, so LDS binds first.
+# MI355X: 256 CUs at 2.4 GHz.  LDS: 256 B per CU per clock of ds_read_b128 (16
+# lanes per clock; a probe with little VALU reached 190 B/clk/CU).  VALU: 64 lanes
+# per CU per clock.  The dedispersion sums channel-pair rows: a 4-byte LDS word
+# holds two outputs' sums of a channel pair (one byte per add), and v_add3_u32
+# adds two such words into u16-pair accumulators (eight adds per lane-op).
 LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9
-VALU_PEAK_TADDS = 256 * 64 * 4 * 2.4e9 / 1e12
+VALU_PEAK_TADDS = 256 * 64 * 8 * 2.4e9 / 1e12
+LDS_BYTES_PER_ADD = 1.0
 
 
 def log(*a):
@@ -199,12 +201,12 @@ def main():
         log("top-20 leaders differing (rank, fp16, fp32): %s" % diff[:6])
     snr_err = max([abs(x["score"] - y["score"]) / abs(y["score"]) for x, y in zip(fam16, fam32)] or [0.0])
 
-    # dominant kernel: dedispersion (one packed 16-bit add per channel and output;
-    # the add's operand comes from LDS, 2 bytes per add)
+    # dominant kernel: dedispersion (nchans adds per output; their operands come
+    # from LDS as channel-pair words, one byte per add)
     dd_ms = stages["dedisperse_ms"]
     adds = float(units) * w.nchans
     ach_tadds = adds / (dd_ms * 1e-3) / 1e12
-    ach_lds = adds * 2 / (dd_ms * 1e-3) / 1e9
+    ach_lds = adds * LDS_BYTES_PER_ADD / (dd_ms * 1e-3) / 1e9
     tpu = traffic_per_unit()
     launches = int(stages.get("launches", 0)) or 1
     units_per_launch = float(units) / launches
@@ -238,8 +240,8 @@ def main():
                      "traffic": dd_traffic * units_per_launch if dd_traffic else None,
                      "valu": {"achieved": ach_tadds, "peak": VALU_PEAK_TADDS, "unit": "Tadd/s",
                               "frac": ach_tadds / VALU_PEAK_TADDS},
-                     "note": "2 LDS bytes per add, adds = trials x series_len x nchans; HIP-event time of "
-                             "the dedispersion stage; traffic = HBM bytes per launch"},
+                     "note": "1 LDS byte per add (channel-pair rows), adds = trials x series_len x nchans; "
+                             "HIP-event time of the dedispersion stage; traffic = HBM bytes per launch"},
         "roofline_spectral": {"bound": "hbm", "kernels": "fft+whiten+harmsum",
                               "achieved": float(units) * spec_alg / (spec_ms * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -86,7 +86,9 @@ constexpr uint32_t DEDISP_BLOCK_TRIALS = 64;
 // spread[c] (host, per channel) bounds the delay spread of channel c over any
 // block of DEDISP_BLOCK_TRIALS trials.  Returns false (and launches nothing)
 // when the spreads need more LDS than a workgroup has or the totals could pass
-// 2^20; the caller then uses launch_dedisp_f32.
+// 2^20; the caller then uses launch_dedisp_f32.  `table` holds
+// dedisp_u8_table_words(nchans, ntrials) words: the blocks' channel bases and
+// the per-block plans (stages of channel-pair rows) a planner kernel writes.
 bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,

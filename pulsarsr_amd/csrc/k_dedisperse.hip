@@ -16,8 +16,6 @@
 
 namespace psr {
 
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-
 // ---------------------------------------------------------------- ingest
 // Packed time-major SIGPROC frames -> channel-major rows (u8 for <=8 bits,
 // u16 for 16 bits, f32 for 32 bits).  Sample i = t*nchans + ch occupies bits
@@ -143,28 +141,26 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // ------------------------------------------------- integer dedispersion (u8)
 // Workgroup = 8 waves sharing one time tile of DD_TILE = 512 outputs and a block
 // of DD_TRIALS = 64 consecutive (ascending) DM trials; each wave owns DD_M = 8
-// trials.  Channels are staged a few at a time into LDS as 32-bit words
+// trials.  A channel's samples are staged into LDS as 32-bit words
 //     w[i] = s[base + i] | s[base + i + 256] << 16,
 // with base = the block's smallest delay of the channel rounded down to 4.  A
 // trial whose delay is base + e needs words e .. e + 255; lane l takes words
 // e + 4l .. e + 4l + 3 with one 16-byte ds_read_b128.  ds_read_b128 needs a
-// 16-byte aligned address, so every channel row is staged four times, copy k
-// starting k words later; the trial reads copy e & 3 at word e & ~3.  The 64
-// lanes of a wave read 1 KiB of consecutive LDS, free of bank conflicts.  Rows
-// are W4_c words long, from channel c's own delay spread over the block (low
-// frequencies spread most), so a stage holds as many channels as fit.
+// 16-byte aligned address, so every row is staged four times, copy k starting
+// k words later; the trial reads copy e & 3 at word e & ~3.  The 64 lanes of a
+// wave read 1 KiB of consecutive LDS, free of bank conflicts.  A row is only as
+// long as the delays of the trials reading it require.
 //
-// Channels are added in pairs with v_add3_u32 on the u16 pairs: within a
-// "phase" of at most 65535 / maxval channels no half can carry, so the 32-bit
-// add is two exact 16-bit adds.  At each phase end the halves are added into
-// running sums mod 2^16 whose carries are counted in 4-bit fields, so totals up
-// to 2^20 are exact.
+// Rows are added in pairs with v_add3_u32 on the u16 pairs: within a "phase"
+// of at most 65535 / (2 maxval) rows no half can carry, so the 32-bit add is
+// two exact 16-bit adds.  At each phase end the halves are added into running
+// sums mod 2^16 whose carries are counted in 4-bit fields, so totals up to 2^20
+// are exact.
 //
-// Staging is pipelined: while a stage is summed, each wave loads the raw bytes
-// and trial offsets of one channel of the next stage straight into LDS (MUBUF
-// loads with the LDS bit, no registers held); at the stage boundary the wave
-// turns them into the four row copies and the per-trial LDS offsets (16 bits
-// each; a wave reads its 8 offsets of a channel with one 16-byte broadcast read).
+// Staging is pipelined: while a stage is summed, the raw bytes of the next
+// stage's channels (one channel per wave) and the record of the stage after it
+// are loaded straight into LDS (MUBUF loads with the LDS bit, no registers
+// held); at the stage boundary the waves turn the raw bytes into the rows.
 //
 // Workgroups are numbered XCD-aware: consecutive workgroup ids go to different
 // XCDs, so id is remapped so that each XCD walks a contiguous run of
@@ -180,66 +176,225 @@ constexpr int DD_XCDS = 8;
 constexpr int DD_AHEAD = 4;  // trials whose LDS reads are issued ahead of their adds
 constexpr int DD_ROW_BYTES = 56 * 1024;  // row copies of one stage (at most)
 constexpr int DD_RAW_BYTES = 8192;       // raw byte windows of one stage (at most)
-constexpr int DD_RAW_CHANS = DD_WAVES;   // channels per stage (at most): one per wave
 constexpr int DD_RSRC_WORD3 = 0x00020000;  // buffer descriptor word 3: raw 32-bit data
-
-__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
-    u16x2 x = __builtin_bit_cast(u16x2, a), y = __builtin_bit_cast(u16x2, b);
-    return __builtin_bit_cast(uint32_t, x + y);
-}
-
-// bases[blk][c] = delay of the block's first trial at channel c, rounded down to
-// a multiple of 4; rel[blk][c][m] = delay of trial blk*32 + m minus that base
-// (trials past the end repeat the last trial).
-__global__ __launch_bounds__(256) void k_dd_table(const int32_t* __restrict__ delays,
-                                                  uint32_t nchans, uint32_t ntrials,
-                                                  int32_t* __restrict__ bases,
-                                                  int32_t* __restrict__ rel) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
-    if (i >= (uint64_t)nblk * nchans) return;
-    const uint32_t blk = (uint32_t)(i / nchans), c = (uint32_t)(i % nchans);
-    const uint32_t d0 = blk * DD_TRIALS;
-    const int32_t base = delays[(size_t)d0 * nchans + c] & ~3;
-    bases[i] = base;
-    for (int m = 0; m < DD_TRIALS; m++) {
-        const uint32_t d = min(d0 + m, ntrials - 1);
-        rel[i * DD_TRIALS + m] = delays[(size_t)d * nchans + c] - base;
-    }
-}
-
-__device__ __forceinline__ uint32_t load_u32(const uint8_t* row, uint64_t off, uint64_t pitch) {
-    return off + 4 <= pitch ? *(const uint32_t*)(row + off) : 0u;
-}
 
 // pair word of byte i (0..3) of windows a (low half) and b (high half)
 __device__ __forceinline__ uint32_t pair_word(uint32_t b, uint32_t a, int i) {
     return __builtin_amdgcn_perm(b, a, 0x0c040c00u + 0x00010001u * (uint32_t)i);
 }
 
-__global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
-    const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_chans,
-    const int32_t* __restrict__ bases, const int32_t* __restrict__ rel, uint32_t ntrials,
-    uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out, uint32_t out_pitch,
-    int fp16, float m0, float inv_s0, double* __restrict__ partial,
-    const uint32_t* __restrict__ cinfo, const uint32_t* __restrict__ sfirst, uint32_t nstages,
-    uint32_t RB) {
-    // Static LDS: the stage's row copies, the [stage channel][DD_TRIALS] LDS byte
-    // offsets of the runs each trial reads, and the next stage's raw bytes
-    // [channel][RB] and trial offsets [channel][DD_TRIALS] written by direct
-    // global->LDS loads.  They are separate LDS objects, so the compiler can tell
-    // that the sums' reads of the rows need not wait for those loads.  Dynamic
-    // LDS: per channel its base, its row length W4 and row offset (cinfo), then
-    // the stages' first channels.
-    __shared__ __align__(16) uint32_t lds[DD_ROW_BYTES / 4];
-    __shared__ __align__(16) uint16_t laddr[DD_RAW_CHANS * DD_TRIALS];
-    __shared__ __align__(16) uint8_t raw[DD_RAW_BYTES];
-    __shared__ __align__(16) int32_t rrel[DD_RAW_CHANS * DD_TRIALS];
-    extern __shared__ __align__(16) int32_t lbase[];
-    uint32_t* lcinfo = (uint32_t*)(lbase + nchans);
-    uint32_t* lsfirst = lcinfo + nchans;
+// Channels are summed in pairs (c, c + 1).  Within a block of DD_TRIALS
+// trials the relative delay delta = delay[c + 1] - delay[c] takes few distinct
+// values (two or three when the block spans a small DM range), so for each
+// value the workgroup builds one "pair row" of words
+//     P[i] = w_c[i] + w_(c+1)[i + delta - (base_(c+1) - base_c)]
+// (u16 halves of at most 2 * 255) and every trial reads one pair row per
+// channel pair: half the LDS reads and half the adds of reading both channel
+// rows.  Pairs whose delta takes more than kmax values in a block are summed as
+// two single-channel rows.
+//
+// A planner kernel lays out each block as stages: consecutive slots (a slot is
+// a channel pair or a single channel) whose rows fit the LDS row area and
+// whose raw windows (one per channel) fit the raw area, one window per wave.
+// A stage is one fixed-size record -- header, row records and every trial's
+// 16-bit LDS read address per slot -- that the dedispersion kernel copies
+// into LDS two stages ahead with direct global->LDS loads, so the stage loop
+// issues no scalar loads (whose out-of-order returns would force full LDS
+// waits in the sums).
+constexpr int DDP_MAX_ROWS = 2 * DD_WAVES;  // wave w builds rows w and w + DD_WAVES
+constexpr int DDP_MAX_SLOTS = DD_WAVES;     // slots per stage
+constexpr int DDP_KMAX = 3;                 // pair rows per channel pair (at most)
+// stage record (u32 words): [0] nwin | nslot << 8 | nrow << 16, [4..11] window
+// channels, [16 + 4 r ..] row r: rowoff | quads << 16, raw byte of word L of
+// channel c in its window, the same for channel c + 1, cwin | c1win << 8 |
+// paired << 16; [80 + 32 s ..] read addresses of slot s (DD_TRIALS x u16)
+constexpr int DDP_ROW_REC = 16;
+constexpr int DDP_LAD_REC = DDP_ROW_REC + 4 * DDP_MAX_ROWS;
+constexpr int DDP_STAGE_WORDS = DDP_LAD_REC + DDP_MAX_SLOTS * DD_TRIALS / 2;  // 336
+constexpr int DDP_STAGE_CHUNKS = (DDP_STAGE_WORDS + 63) / 64;                   // 64-word loads
+constexpr int DDP_STAGE_PITCH = 64 * DDP_STAGE_CHUNKS;                          // 384 words
+constexpr int DDP_PLAN_HEAD = 64;  // word 0: stage count
 
-    // XCD-aware work index (bijective over total = ntiles * nblk)
+static size_t ddp_plan_words(uint32_t nchans) {  // per block; stages <= slots <= nchans
+    return DDP_PLAN_HEAD + (size_t)nchans * DDP_STAGE_PITCH;
+}
+
+// Raw window of channel ch in a tile starting at t0: bytes from
+// t0 + (base & ~15) - 16, so sample t0 + base + i sits at byte mis + i with
+// mis = (base & 15) + 16 (at least 13 bytes of margin below any word a pair
+// row of channel c + 1 reads).
+__device__ __forceinline__ uint32_t ddp_mis(int32_t base) { return (uint32_t)(base & 15) + 16u; }
+
+// One workgroup per block of DD_TRIALS trials.  Dynamic LDS: bases[nchans],
+// per slot its stage | position << 16, then per channel pair its kind (K pair
+// rows, 0 = single-channel rows) and first slot, and per row its delta,
+// L | W << 16 and LDS offset.
+__global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ delays, uint32_t nchans,
+                                                 uint32_t ntrials, uint32_t kmax, uint32_t max_windows,
+                                                 uint32_t row_budget, int32_t* __restrict__ bases,
+                                                 uint32_t* __restrict__ plan, uint64_t plan_words) {
+    extern __shared__ int32_t sbase[];
+    const uint32_t npair = (nchans + 1) / 2;
+    uint32_t* sloc = (uint32_t*)(sbase + nchans);
+    uint32_t* ukind = sloc + nchans;
+    uint32_t* uslot = ukind + npair;
+    int32_t* udel = (int32_t*)(uslot + npair);
+    uint32_t* uLW = (uint32_t*)(udel + (size_t)npair * DDP_KMAX);
+    uint32_t* uRO = uLW + (size_t)npair * DDP_KMAX;
+    const uint32_t blk = blockIdx.x, d0 = blk * DD_TRIALS;
+    uint32_t* pb = plan + (size_t)blk * plan_words;
+    auto delay = [&](uint32_t m, uint32_t c) {
+        return delays[(size_t)min(d0 + m, ntrials - 1) * nchans + c];
+    };
+    for (uint32_t c = threadIdx.x; c < nchans; c += blockDim.x) {
+        const int32_t b = delay(0, c) & ~3;
+        sbase[c] = b;
+        bases[(size_t)blk * nchans + c] = b;
+    }
+    __syncthreads();
+    // channel pairs: distinct deltas and, per delta, the e range of channel c
+    for (uint32_t u = threadIdx.x; u < npair; u += blockDim.x) {
+        const uint32_t c = 2 * u;
+        uint32_t K = 0;
+        int32_t del[DDP_KMAX], lo[DDP_KMAX], hi[DDP_KMAX];
+        if (c + 1 < nchans) {
+            for (uint32_t m = 0; m < (uint32_t)DD_TRIALS; m++) {
+                const int32_t dc = delay(m, c), dl = delay(m, c + 1) - dc, e = dc - sbase[c];
+                uint32_t k = 0;
+                while (k < K && del[k] != dl) k++;
+                if (k == K) {
+                    if (K == kmax) {
+                        K = 0;
+                        break;
+                    }
+                    del[K] = dl;
+                    lo[K] = hi[K] = e;
+                    K++;
+                } else {
+                    lo[k] = min(lo[k], e);
+                    hi[k] = max(hi[k], e);
+                }
+            }
+        }
+        ukind[u] = K;
+        for (uint32_t k = 0; k < K; k++) {
+            const uint32_t L = (uint32_t)lo[k] & ~3u, W = ((uint32_t)hi[k] & ~3u) - L + DD_TILE / 2;
+            udel[u * DDP_KMAX + k] = del[k];
+            uLW[u * DDP_KMAX + k] = L | W << 16;
+        }
+        if (K == 0)  // single-channel rows of c (and c + 1): L = 0
+            for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
+                int32_t emax = 0;
+                for (uint32_t m = 0; m < (uint32_t)DD_TRIALS; m++) emax = max(emax, delay(m, c + j) - sbase[c + j]);
+                uLW[u * DDP_KMAX + j] = (((uint32_t)emax & ~3u) + DD_TILE / 2) << 16;
+            }
+    }
+    __syncthreads();
+    // stages, in channel order
+    if (threadIdx.x == 0) {
+        uint32_t nst = 0, nslot_all = 0;
+        uint32_t nwin = 0, nslot = 0, nrow = 0, bytes = 0;
+        uint32_t* st = pb + DDP_PLAN_HEAD;
+        auto close = [&]() {
+            if (!nslot) return;
+            st[0] = nwin | nslot << 8 | nrow << 16;
+            nst++;
+            st = pb + DDP_PLAN_HEAD + (size_t)nst * DDP_STAGE_PITCH;
+            nwin = nslot = nrow = bytes = 0;
+        };
+        auto put_row = [&](uint32_t ui, uint32_t W, uint32_t offc, uint32_t offc1, uint32_t cw, uint32_t c1w,
+                           uint32_t paired) {
+            uRO[ui] = bytes;
+            uint32_t* r = st + DDP_ROW_REC + 4 * nrow;
+            r[0] = bytes | (W / 4) << 16;
+            r[1] = offc;
+            r[2] = offc1;
+            r[3] = cw | c1w << 8 | paired << 16;
+            bytes += 16 * W;
+            nrow++;
+        };
+        auto put_slot = [&]() {
+            sloc[nslot_all] = nst | nslot << 16;
+            nslot++;
+            nslot_all++;
+        };
+        for (uint32_t u = 0; u < npair; u++) {
+            const uint32_t c = 2 * u, K = ukind[u];
+            if (K) {
+                uint32_t need = 0;
+                for (uint32_t k = 0; k < K; k++) need += 16 * (uLW[u * DDP_KMAX + k] >> 16);
+                if (nwin + 2 > max_windows || nrow + K > (uint32_t)DDP_MAX_ROWS ||
+                    bytes + need > row_budget || nslot == (uint32_t)DDP_MAX_SLOTS)
+                    close();
+                uslot[u] = nslot_all;
+                st[4 + nwin] = c;
+                st[5 + nwin] = c + 1;
+                const int32_t shift = sbase[c + 1] - sbase[c];
+                for (uint32_t k = 0; k < K; k++) {
+                    const uint32_t L = uLW[u * DDP_KMAX + k] & 0xffffu, W = uLW[u * DDP_KMAX + k] >> 16;
+                    const int32_t o1 = (int32_t)ddp_mis(sbase[c + 1]) + (int32_t)L + udel[u * DDP_KMAX + k] - shift;
+                    put_row(u * DDP_KMAX + k, W, ddp_mis(sbase[c]) + L, (uint32_t)o1, nwin, nwin + 1, 1);
+                }
+                nwin += 2;
+                put_slot();
+            } else {
+                for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
+                    const uint32_t W = uLW[u * DDP_KMAX + j] >> 16;
+                    if (nwin + 1 > max_windows || nrow + 1 > (uint32_t)DDP_MAX_ROWS ||
+                        bytes + 16 * W > row_budget || nslot == (uint32_t)DDP_MAX_SLOTS)
+                        close();
+                    if (j == 0) uslot[u] = nslot_all;
+                    st[4 + nwin] = c + j;
+                    put_row(u * DDP_KMAX + j, W, ddp_mis(sbase[c + j]), 0, nwin, 0, 0);
+                    nwin++;
+                    put_slot();
+                }
+            }
+        }
+        close();
+        pb[0] = nst;
+    }
+    __syncthreads();
+    // every trial's LDS read address per slot
+    auto lad = [&](uint32_t slot) {
+        const uint32_t loc = sloc[slot];
+        return (uint16_t*)(pb + DDP_PLAN_HEAD + (size_t)(loc & 0xffffu) * DDP_STAGE_PITCH + DDP_LAD_REC) +
+               (size_t)(loc >> 16) * DD_TRIALS;
+    };
+    for (uint32_t idx = threadIdx.x; idx < npair * DD_TRIALS; idx += blockDim.x) {
+        const uint32_t u = idx / DD_TRIALS, m = idx % DD_TRIALS, c = 2 * u, K = ukind[u];
+        if (K) {
+            const int32_t dc = delay(m, c), dl = delay(m, c + 1) - dc;
+            const uint32_t e = (uint32_t)(dc - sbase[c]);
+            uint32_t k = 0;
+            while (k + 1 < K && udel[u * DDP_KMAX + k] != dl) k++;
+            const uint32_t L = uLW[u * DDP_KMAX + k] & 0xffffu, W = uLW[u * DDP_KMAX + k] >> 16;
+            lad(uslot[u])[m] = (uint16_t)(uRO[u * DDP_KMAX + k] + ((e & 3) * W + (e & ~3u) - L) * 4);
+        } else {
+            for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
+                const uint32_t e = (uint32_t)(delay(m, c + j) - sbase[c + j]);
+                const uint32_t W = uLW[u * DDP_KMAX + j] >> 16;
+                lad(uslot[u] + j)[m] = (uint16_t)(uRO[u * DDP_KMAX + j] + ((e & 3) * W + (e & ~3u)) * 4);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
+    const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_slots,
+    const int32_t* __restrict__ bases, const uint32_t* __restrict__ plan, uint64_t plan_words,
+    uint32_t ntrials, uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out,
+    uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t RB) {
+    // Static LDS: the stage's rows, three stage records (the current one, the
+    // next, whose raw windows are loaded during this stage, and the one after,
+    // loaded now) and the raw windows [window][RB]; records and windows arrive
+    // by direct global->LDS loads into separate LDS objects, so the sums'
+    // reads of the rows need not wait for them.  Dynamic LDS: channel bases.
+    __shared__ __align__(16) uint32_t lds[DD_ROW_BYTES / 4];
+    __shared__ __align__(16) uint32_t srec[3][DDP_STAGE_PITCH];
+    __shared__ __align__(16) uint8_t raw[DD_RAW_BYTES];
+    extern __shared__ __align__(16) int32_t lbase[];
+
     const uint32_t total = ntiles * nblk;
     const uint32_t id = blockIdx.x;
     const uint32_t xcd = id % DD_XCDS, local = id / DD_XCDS;
@@ -251,12 +406,14 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     const uint32_t d0 = blk * DD_TRIALS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int32_t* bblk = bases + (size_t)blk * nchans;
-    const int32_t* rblk = rel + (size_t)blk * nchans * DD_TRIALS;
+    const uint32_t* pb = plan + (size_t)blk * plan_words;
+    const uint32_t nstages = __builtin_amdgcn_readfirstlane(pb[0]);
+    // Workgroups start at different stages (the sums are exact in any order),
+    // so that two workgroups sharing a CU are rarely building rows at the
+    // same time.
+    uint32_t rot = id * 0x9e3779b9u;
+    rot = nstages ? (rot >> 16) % nstages : 0;
 
-    // acc16: this phase's sums (u16 pairs, no half can overflow within a phase);
-    // lo: running sums mod 2^16 per half; cy: carries out of each half, 4 bits per
-    // output (bits 4j + 16h for word j, half h; the host keeps every total below 2^20)
     uint32_t acc16[DD_M][4], lo[DD_M][4], cy[DD_M];
 #pragma unroll
     for (int m = 0; m < DD_M; m++) {
@@ -279,91 +436,129 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     };
     uint32_t in_phase = 0;
     const char* lds_lane = (const char*)lds + 16 * lane;
-    for (uint32_t i = threadIdx.x; i < nchans; i += DD_THREADS) {
-        lbase[i] = bblk[i];
-        lcinfo[i] = cinfo[i];
-    }
-    for (uint32_t i = threadIdx.x; i <= nstages; i += DD_THREADS) lsfirst[i] = sfirst[i];
-    // Wave w stages channel w of each stage.  Its raw window starts at t0 + base,
-    // read from the 16-byte aligned start s16 in 1 KiB wave loads into raw slot w;
-    // lanes past the row end read its last 16 bytes instead (padding, only under
-    // outputs at or past n).  MUBUF loads (not FLAT) keep the sums' waits partial.
-    auto issue = [&](uint32_t st) {
-        const uint32_t cc = __builtin_amdgcn_readfirstlane(lsfirst[st]);
-        const uint32_t nc = __builtin_amdgcn_readfirstlane(lsfirst[st + 1]) - cc;
-        if ((uint32_t)wave >= nc) return;
+    for (uint32_t i = threadIdx.x; i < nchans; i += DD_THREADS) lbase[i] = bases[(size_t)blk * nchans + i];
+    // record of the st-th stage taken into srec[st % 3]: wave w < DDP_STAGE_CHUNKS copies 64 words
+    auto load_record = [&](uint32_t st) {
+        if ((uint32_t)wave >= (uint32_t)DDP_STAGE_CHUNKS) return;
         uint32_t ln = threadIdx.x;  // recomputed per call: hoisted per-lane values would spill
         asm volatile("" : "+v"(ln));
         ln &= 63;
-        const uint32_t c = cc + wave;
-        const uint32_t w4 = __builtin_amdgcn_readfirstlane(lcinfo[c]) & 0xffffu;
-        const uint32_t s16 = ((uint32_t)t0 + (uint32_t)__builtin_amdgcn_readfirstlane(lbase[c])) & ~15u;
+        uint32_t sx = st + rot;
+        sx = sx >= nstages ? sx - nstages : sx;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(pb + DDP_PLAN_HEAD + (size_t)sx * DDP_STAGE_PITCH), 0, 4 * DDP_STAGE_PITCH, DD_RSRC_WORD3);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, srec[st % 3] + 64 * wave, 4, 4 * (64 * wave + ln), 0, 0, 0);
+    };
+    // raw windows of stage st: wave w < nwin loads channel w's window (lanes
+    // whose 16 bytes fall outside the row read its nearest 16 in-row bytes
+    // instead: those samples only reach outputs at or past n, or words no
+    // trial reads)
+    auto load_windows = [&](uint32_t st) {
+        const uint32_t* sr = srec[st % 3];
+        const uint32_t nwin = __builtin_amdgcn_readfirstlane(sr[0]) & 0xffu;
+        if ((uint32_t)wave >= nwin) return;
+        uint32_t ln = threadIdx.x;
+        asm volatile("" : "+v"(ln));
+        ln &= 63;
+        const uint32_t c = __builtin_amdgcn_readfirstlane(sr[4 + wave]);
+        const int64_t start = (int64_t)t0 + (int64_t)(lbase[c] & ~15) - 16;
         const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(data + (size_t)c * pitch), 0, (int)pitch, DD_RSRC_WORD3);
-        for (uint32_t k = 0; k < w4 + 276; k += 1024) {
-            const uint32_t p = (uint32_t)min((uint64_t)s16 + k + 16u * ln, pitch - 16);
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, raw + (size_t)wave * RB + k, 16, p, 0, 0, 0);
+        for (uint32_t k = 0; k < RB; k += 1024) {
+            int64_t p = start + (int64_t)k + 16 * (int64_t)ln;
+            p = p < 0 ? 0 : (p > (int64_t)pitch - 16 ? (int64_t)pitch - 16 : p);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, raw + (size_t)wave * RB + k, 16, (uint32_t)p, 0, 0, 0);
         }
-        const __amdgpu_buffer_rsrc_t rt = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(rblk + (size_t)c * DD_TRIALS), 0, 4 * DD_TRIALS, DD_RSRC_WORD3);
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rt, rrel + (size_t)wave * DD_TRIALS, 4, 4 * ln, 0, 0, 0);
     };
+    load_record(0);
+    if (nstages > 1) load_record(1);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
-    issue(0);
+    load_windows(0);
     for (uint32_t st = 0; st < nstages; st++) {
-        const uint32_t cc = __builtin_amdgcn_readfirstlane(lsfirst[st]);
-        const uint32_t nc = __builtin_amdgcn_readfirstlane(lsfirst[st + 1]) - cc;
-        // this wave's loads have landed; after the barrier every wave's have, and the
-        // previous stage's sums are done with the rows
+        const uint32_t* sr = srec[st % 3];
+        const uint32_t h = __builtin_amdgcn_readfirstlane(sr[0]);
+        const uint32_t nslot = (h >> 8) & 0xffu, nrow = h >> 16;
+        // this wave's loads (stage st's windows, stage st + 1's record) have
+        // landed; after the barrier every wave's have, and the previous stage's
+        // sums are done with the rows
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         __syncthreads();
-        if ((uint32_t)wave < nc) {
-            uint32_t ln = threadIdx.x;  // as in issue
+        // rows: wave w builds rows w and w + DD_WAVES of the stage
+        for (uint32_t rr = (uint32_t)wave; rr < nrow; rr += DD_WAVES) {
+            uint32_t ln = threadIdx.x;  // as in load_record
             asm volatile("" : "+v"(ln));
             ln &= 63;
-            const uint32_t c = cc + wave;
-            const uint32_t info = __builtin_amdgcn_readfirstlane(lcinfo[c]);
-            const uint32_t w4 = info & 0xffffu, rowoff = info >> 16, quads = w4 / 4;
-            const uint32_t mis = ((uint32_t)t0 + (uint32_t)__builtin_amdgcn_readfirstlane(lbase[c])) & 15u;
-            const uint32_t* rw0 = (const uint32_t*)(raw + (size_t)wave * RB + mis);
-            uint32_t* row = lds + rowoff / 4;
-            for (uint32_t qd = ln; qd < quads; qd += 64) {
-                const uint32_t* rw = rw0 + qd;
-                const uint32_t a0 = rw[0], a1 = rw[1], b0 = rw[64], b1 = rw[65];
-                uint32_t* dst = row + 4 * qd;
-                // w[k] pairs byte k of the 8-byte window at 4 qd with byte k of the
-                // window 256 bytes later; copy k holds w[k .. k + 3]
-                uint32_t w[7];
+            const uint32_t* rec = sr + DDP_ROW_REC + 4 * rr;
+            const uint32_t r0 = __builtin_amdgcn_readfirstlane(rec[0]);
+            const uint32_t offc = __builtin_amdgcn_readfirstlane(rec[1]);
+            const uint32_t offc1 = __builtin_amdgcn_readfirstlane(rec[2]);
+            const uint32_t r3 = __builtin_amdgcn_readfirstlane(rec[3]);
+            const uint32_t quads = r0 >> 16, W = 4 * quads;
+            uint32_t* row = lds + (r0 & 0xffffu) / 4;
+            const uint32_t* rc = (const uint32_t*)(raw + (size_t)(r3 & 0xffu) * RB + offc);
+            if (!(r3 >> 16)) {
+                for (uint32_t qd = ln; qd < quads; qd += 64) {
+                    const uint32_t* rw = rc + qd;
+                    const uint32_t a0 = rw[0], a1 = rw[1], b0 = rw[64], b1 = rw[65];
+                    uint32_t* dst = row + 4 * qd;
+                    uint32_t w[7];
 #pragma unroll
-                for (int k = 0; k < 4; k++) w[k] = pair_word(b0, a0, k);
+                    for (int k = 0; k < 4; k++) w[k] = pair_word(b0, a0, k);
 #pragma unroll
-                for (int k = 0; k < 3; k++) w[4 + k] = pair_word(b1, a1, k);
+                    for (int k = 0; k < 3; k++) w[4 + k] = pair_word(b1, a1, k);
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    *(uint4*)(dst + (size_t)k * w4) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+                    for (int k = 0; k < 4; k++)
+                        *(uint4*)(dst + (size_t)k * W) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+                }
+            } else {
+                const uint32_t* r1 = (const uint32_t*)(raw + (size_t)((r3 >> 8) & 0xffu) * RB + (offc1 & ~3u));
+                const uint32_t sh = offc1 & 3u;
+                for (uint32_t qd = ln; qd < quads; qd += 64) {
+                    const uint32_t* rw = rc + qd;
+                    const uint32_t a0 = rw[0], a1 = rw[1], b0 = rw[64], b1 = rw[65];
+                    const uint32_t* rv = r1 + qd;
+                    const uint32_t p0 = rv[0], p1 = rv[1], p2 = rv[2], q0 = rv[64], q1 = rv[65], q2 = rv[66];
+                    const uint32_t c0 = __builtin_amdgcn_alignbyte(p1, p0, sh);
+                    const uint32_t c1 = __builtin_amdgcn_alignbyte(p2, p1, sh);
+                    const uint32_t e0 = __builtin_amdgcn_alignbyte(q1, q0, sh);
+                    const uint32_t e1 = __builtin_amdgcn_alignbyte(q2, q1, sh);
+                    uint32_t* dst = row + 4 * qd;
+                    uint32_t w[7];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) w[k] = pair_word(b0, a0, k) + pair_word(e0, c0, k);
+#pragma unroll
+                    for (int k = 0; k < 3; k++) w[4 + k] = pair_word(b1, a1, k) + pair_word(e1, c1, k);
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        *(uint4*)(dst + (size_t)k * W) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+                }
             }
-            const int32_t e = rrel[wave * DD_TRIALS + ln];
-            laddr[wave * DD_TRIALS + ln] = (uint16_t)(rowoff + ((uint32_t)(e & 3) * w4 + (uint32_t)(e & ~3)) * 4);
         }
         __syncthreads();
-        if (st + 1 < nstages) issue(st + 1);
-        // channels in pairs: one v_add3_u32 adds both channels' u16 pairs (no half
-        // carries within a phase, so the 32-bit add is two exact 16-bit adds).  The
-        // wave's 8 offsets per channel come from 16-byte broadcast reads (16 bits
-        // each), fetched one pair ahead.
-        const uint16_t* lw = laddr + wave * DD_M;
+        if (st + 1 < nstages) load_windows(st + 1);
+        if (st + 2 < nstages) load_record(st + 2);
+        // slots in pairs: one v_add3_u32 adds both slots' u16 pairs (no half can
+        // carry within a phase: every slot adds at most 2 * maxval per half).
+        // The wave's 8 read addresses per slot come from 16-byte broadcast reads
+        // (16 bits each), fetched one pair ahead.
+        const uint16_t* lw = (const uint16_t*)(sr + DDP_LAD_REC) + wave * DD_M;
         uint4 A = *(const uint4*)(lw), B = *(const uint4*)(lw + DD_TRIALS);
-        for (uint32_t c = 0; c + 1 < nc; c += 2) {
-            const uint32_t pa[4] = {A.x, A.y, A.z, A.w}, pb[4] = {B.x, B.y, B.z, B.w};
+        for (uint32_t c = 0; c + 1 < nslot; c += 2) {
+            if (in_phase + 2 > phase_slots) {
+                widen();
+                in_phase = 0;
+            }
+            const uint32_t pa[4] = {A.x, A.y, A.z, A.w}, pb2[4] = {B.x, B.y, B.z, B.w};
             uint4 x[DD_M], y[DD_M];
 #pragma unroll
             for (int m = 0; m < DD_M; m++) {
                 const uint32_t sa = m & 1 ? pa[m >> 1] >> 16 : pa[m >> 1] & 0xffffu;
-                const uint32_t sb = m & 1 ? pb[m >> 1] >> 16 : pb[m >> 1] & 0xffffu;
+                const uint32_t sb = m & 1 ? pb2[m >> 1] >> 16 : pb2[m >> 1] & 0xffffu;
                 x[m] = *(const uint4*)(lds_lane + sa);
                 y[m] = *(const uint4*)(lds_lane + sb);
             }
-            const uint32_t cn = c + 3 < nc ? c + 2 : c;  // next pair (a repeat at the end)
+            const uint32_t cn = c + 3 < nslot ? c + 2 : c;  // next pair (a repeat at the end)
             A = *(const uint4*)(lw + cn * DD_TRIALS);
             B = *(const uint4*)(lw + (cn + 1) * DD_TRIALS);
 #pragma unroll
@@ -374,7 +569,7 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
                 acc16[m][3] = acc16[m][3] + x[m].w + y[m].w;
             }
             // issue order: the reads of DD_AHEAD trials, then each trial's four adds
-            // followed by the two reads DD_AHEAD trials later; the next offsets go
+            // followed by the two reads DD_AHEAD trials later; the next addresses go
             // out after the last data reads
 #pragma unroll
             for (int m = 0; m < DD_AHEAD; m++) {
@@ -391,15 +586,14 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
                 if (m + DD_AHEAD == DD_M - 1) __builtin_amdgcn_sched_group_barrier(0x0100, 2, 0);
             }
             in_phase += 2;
-            if (in_phase == phase_chans) {
-                in_phase = 0;
-                widen();
-            }
         }
-        if (nc & 1) {  // odd channel count: the last stage's final channel alone
-            const uint32_t c = nc - 1;
-            const uint4 A = *(const uint4*)(laddr + c * DD_TRIALS + wave * DD_M);
-            const uint32_t pa[4] = {A.x, A.y, A.z, A.w};
+        if (nslot & 1) {  // odd slot count: the stage's last slot alone
+            if (in_phase + 1 > phase_slots) {
+                widen();
+                in_phase = 0;
+            }
+            const uint4 A2 = *(const uint4*)(lw + (nslot - 1) * DD_TRIALS);
+            const uint32_t pa[4] = {A2.x, A2.y, A2.z, A2.w};
 #pragma unroll
             for (int m = 0; m < DD_M; m++) {
                 const uint32_t sa = m & 1 ? pa[m >> 1] >> 16 : pa[m >> 1] & 0xffffu;
@@ -409,7 +603,7 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
                 acc16[m][2] += x.z;
                 acc16[m][3] += x.w;
             }
-            in_phase += 1;  // phase_chans is even and in_phase was even: no boundary here
+            in_phase += 1;
         }
     }
     widen();
@@ -470,69 +664,55 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
 
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
-    // bases, relative delays, then the channels' row info and the stages' first channels
-    return nblk * nchans * (1 + DD_TRIALS) + 2 * (size_t)nchans + 2;
+    return nblk * nchans + nblk * ddp_plan_words(nchans);  // bases, then the plans
 }
 
 bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
-                      uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
-                      uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
-                      double* partial, const uint32_t* spread) {
+                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
+                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                       double* partial, const uint32_t* spread) {
     // 4-bit carry counters per output: every total must stay below 2^20;
-    // 16-byte aligned raw loads inside each row; 32-bit buffer offsets
-    if (nchans == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
-    if (pitch % 16 || pitch < 16 || pitch >= (1ull << 31)) return false;
-    // Row length of channel c: a trial reads words (e & ~3) .. (e & ~3) + 255 with
-    // e <= spread[c] + 3.  Its raw window (read from a 16-byte aligned start) takes
-    // W4 + 276 bytes, in whole 1 KiB wave loads; RB is the raw slot size.
-    std::vector<uint32_t> w4(nchans), info(nchans), first;
-    uint64_t RB = 1024;
-    for (uint32_t c = 0; c < nchans; c++) {
-        const uint64_t w = ((uint64_t)spread[c] + 3 + 3) / 4 * 4 + DD_TILE / 2;
-        if (w > 0xffff) return false;
-        w4[c] = (uint32_t)w;
-        RB = std::max<uint64_t>(RB, (w + 276 + 1023) / 1024 * 1024);
-    }
-    if (RB * 2 > DD_RAW_BYTES) return false;
-    const uint32_t slots = std::min<uint32_t>(DD_RAW_CHANS, (uint32_t)(DD_RAW_BYTES / RB));
-    // stages: consecutive channels whose four row copies fit DD_ROW_BYTES (< 64 KiB,
-    // for the 16-bit offsets), one per wave; an even count (channels are added in
-    // pairs) except in the last stage
-    for (uint32_t c = 0; c < nchans;) {
-        uint32_t k = 0, rows = 0;
-        while (c + k < nchans && k < slots && rows + 16 * w4[c + k] <= (uint32_t)DD_ROW_BYTES)
-            rows += 16 * w4[c + k++];
-        if (c + k < nchans) k &= ~1u;
-        if (k == 0) return false;
-        first.push_back(c);
-        for (uint32_t j = 0, off = 0; j < k; off += 16 * w4[c + j], j++) info[c + j] = w4[c + j] | off << 16;
-        c += k;
-    }
-    const uint32_t nstages = (uint32_t)first.size();
-    first.push_back(nchans);
-    // dynamic LDS: bases and row info per channel, first channel per stage
-    const size_t shm = 8 * (size_t)nchans + 4 * (size_t)(nstages + 1);
-    if (shm > 80 * 1024) return false;
+    // 16-byte raw loads; 32-bit buffer offsets; 16-bit stage numbers
+    if (nchans == 0 || ntrials == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
+    if (pitch % 16 || pitch < 16 || pitch >= (1ull << 31) || nchans > 0xffff) return false;
+    // A row has W <= spread + 262 words (a trial reads words (e & ~3) - L ..
+    // + 255 of each copy); its raw windows are read up to byte spread + 556.
+    uint32_t smax = 0;
+    for (uint32_t c = 0; c < nchans; c++) smax = std::max(smax, spread[c]);
+    const uint64_t wmax = (uint64_t)smax + 3 + 3 + DD_TILE / 2;
+    const uint32_t RB = (uint32_t)(((uint64_t)smax + 600 + 1023) / 1024 * 1024);
+    if (RB > DD_RAW_BYTES) return false;
+    const uint32_t windows = std::min<uint32_t>(DD_WAVES, DD_RAW_BYTES / RB);
+    if (windows < 2 || 16 * wmax > (uint64_t)DD_ROW_BYTES) return false;
+    // a slot of kmax pair rows must fit the row area alone
+    uint32_t kmax = DDP_KMAX;
+    while (kmax > 1 && (uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax--;
+    if ((uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax = 0;
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
+    const size_t npair = (nchans + 1) / 2;
+    const size_t plan_shm = 4 * (2 * (size_t)nchans + 2 * npair + 3 * DDP_KMAX * npair);
+    if (plan_shm > 160 * 1024) return false;
+    const size_t shm = 4 * (size_t)nchans;
+    if (shm > 64 * 1024) return false;
     int32_t* bases = table;
-    int32_t* rel = table + nblk * nchans;
-    uint32_t* cinfo = (uint32_t*)(rel + nblk * nchans * DD_TRIALS);
-    uint32_t* sfirst = cinfo + nchans;
-    if (hipMemcpyAsync(cinfo, info.data(), (size_t)nchans * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(sfirst, first.data(), first.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess)
-        return false;
-    hipLaunchKernelGGL(k_dd_table, dim3((unsigned)((nblk * nchans + 255) / 256)), dim3(256), 0, s,
-                       delays, nchans, ntrials, bases, rel);
-    // channels per phase: no 16-bit half can overflow; even, so pairs never straddle
-    uint32_t phase = 65535u / (maxval ? maxval : 1);
-    phase = (phase > 4096 ? 4096 : phase) & ~1u;
+    uint32_t* plan = (uint32_t*)(table + nblk * nchans);
+    const uint64_t pw = ddp_plan_words(nchans);
+    if (plan_shm > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dd_plan),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan_shm);
+    hipLaunchKernelGGL(k_dd_plan, dim3((unsigned)nblk), dim3(256), plan_shm, s, delays, nchans, ntrials,
+                       kmax, windows, (uint32_t)DD_ROW_BYTES, bases, plan, pw);
+    // slots per phase: no 16-bit half can overflow (a slot adds at most 2 * maxval)
+    uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
+    phase = phase > 4096 ? 4096 : phase;
+    if (phase < 2) return false;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dedisp_u8),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     const uint32_t ntiles = (n + DD_TILE - 1) / DD_TILE;
     const uint64_t total = (uint64_t)ntiles * nblk;
     hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(DD_THREADS), shm, s, data, pitch, nchans,
-                       phase, bases, rel, ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16,
-                       m0, inv_s0, partial, cinfo, sfirst, nstages, (uint32_t)RB);
+                       phase, bases, plan, pw, ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16,
+                       m0, inv_s0, partial, RB);
     return true;
 }
 
