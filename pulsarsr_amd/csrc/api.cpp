@@ -219,7 +219,7 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
         (rc = ensure(c, c->spec_t, (size_t)nt * g.spec_pitch * es)) ||
         (rc = ensure(c, c->raw_nat, (size_t)nt * g.spec_pitch * es)) ||
         (rc = ensure(c, c->scales, (size_t)nt * g.nblocks * sizeof(float))) ||
-        (rc = ensure(c, c->blk_sorted, (size_t)nt * g.spec_pitch * es)) ||
+        (rc = ensure(c, c->blk_sorted, whiten_fused(g) ? 8 : (size_t)nt * g.spec_pitch * es)) ||
         (rc = ensure(c, c->floors, (size_t)nt * sizeof(float))) ||
         (rc = ensure(c, c->fft_partial, (size_t)nt * npart * sizeof(double))) ||
         (rc = ensure(c, c->peaks, (size_t)nt * PEAK_CAP * sizeof(RawPeak))) ||

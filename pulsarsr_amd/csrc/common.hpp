@@ -116,7 +116,8 @@ void fill_costabs(float* host_table);
 constexpr uint64_t FFT_BATCH_BYTES = 1ull << 30;
 uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials);
 // whitening scales + natural-order raw spectrum; `sorted` (spectrum-sized) and
-// `floors` (one float per trial) are scratch
+// `floors` (one float per trial) are scratch, unused when whiten_fused(g)
+bool whiten_fused(const SpecGeom& g);
 void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp16,
                    uint32_t ntrials, const double* partial_sums, uint32_t npartials,
                    void* raw_nat, void* sorted, float* floors, float* scales);

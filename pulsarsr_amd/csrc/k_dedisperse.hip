@@ -408,11 +408,6 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     const int lane = threadIdx.x & 63;
     const uint32_t* pb = plan + (size_t)blk * plan_words;
     const uint32_t nstages = __builtin_amdgcn_readfirstlane(pb[0]);
-    // Workgroups start at different stages (the sums are exact in any order),
-    // so that two workgroups sharing a CU are rarely building rows at the
-    // same time.
-    uint32_t rot = id * 0x9e3779b9u;
-    rot = nstages ? (rot >> 16) % nstages : 0;
 
     uint32_t acc16[DD_M][4], lo[DD_M][4], cy[DD_M];
 #pragma unroll
@@ -437,16 +432,14 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     uint32_t in_phase = 0;
     const char* lds_lane = (const char*)lds + 16 * lane;
     for (uint32_t i = threadIdx.x; i < nchans; i += DD_THREADS) lbase[i] = bases[(size_t)blk * nchans + i];
-    // record of the st-th stage taken into srec[st % 3]: wave w < DDP_STAGE_CHUNKS copies 64 words
+    // record of stage st into srec[st % 3]: wave w < DDP_STAGE_CHUNKS copies 64 words
     auto load_record = [&](uint32_t st) {
         if ((uint32_t)wave >= (uint32_t)DDP_STAGE_CHUNKS) return;
         uint32_t ln = threadIdx.x;  // recomputed per call: hoisted per-lane values would spill
         asm volatile("" : "+v"(ln));
         ln &= 63;
-        uint32_t sx = st + rot;
-        sx = sx >= nstages ? sx - nstages : sx;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(pb + DDP_PLAN_HEAD + (size_t)sx * DDP_STAGE_PITCH), 0, 4 * DDP_STAGE_PITCH, DD_RSRC_WORD3);
+            (void*)(pb + DDP_PLAN_HEAD + (size_t)st * DDP_STAGE_PITCH), 0, 4 * DDP_STAGE_PITCH, DD_RSRC_WORD3);
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, srec[st % 3] + 64 * wave, 4, 4 * (64 * wave + ln), 0, 0, 0);
     };
     // raw windows of stage st: wave w < nwin loads channel w's window (lanes

@@ -193,62 +193,14 @@ __global__ __launch_bounds__(256) void k_sort_blocks(
     for (int c = 0; c < 8; c++) store_keys8<F16>(sorted, o + 8 * c, v + 8 * c);
 }
 
-// Stage 2: one thread per 64-bin block b.  The five sorted blocks b-2 .. b+2
-// of its window are staged in LDS; the exact window median is found by a cut
-// search -- a position p_j in every block with sum p_j = len/2 such that all
-// keys left of the cut are <= all keys right of it; the median is then the
-// smallest key right of the cut.  Block `nsorted` (present when m is a
-// multiple of 64) holds Nyquist alone.
-template <bool F16>
-__global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
-    const void* __restrict__ sorted, const void* __restrict__ spec_t, uint32_t m, uint32_t nbins,
-    uint32_t nblocks, uint64_t spec_pitch, const float* __restrict__ floors,
-    float* __restrict__ scales) {
-    // FP16 keys are staged as 16-bit values (66 per block: an odd word stride),
-    // which halves the footprint and doubles the workgroups per CU
-    using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
-    constexpr int STRIDE = F16 ? 2 * 33 : (int)MB_STRIDE;
-    __shared__ KeyT s[(MB_BLOCKS + 4) * STRIDE];
-    constexpr int E = F16 ? 8 : 4;                                  // keys per 16-byte load
-    constexpr int NV = (int)((MB_BLOCKS + 4) * 64 / E);             // vectors staged
-    constexpr int VPT = (NV + (int)MB_BLOCKS - 1) / (int)MB_BLOCKS;  // per thread
-    const uint2 wb = xcd_block();
-    const uint32_t trial = wb.y;
-    const uint32_t b0 = wb.x * MB_BLOCKS;
-    const size_t srow = (size_t)trial * spec_pitch;
-    const uint32_t nsorted = (m + 63) / 64;
-    uint4 q[VPT];
-#pragma unroll
-    for (int t = 0; t < VPT; t++) {
-        const int v = threadIdx.x + MB_BLOCKS * t;
-        const int64_t gb = (int64_t)b0 - 2 + (v * E) / 64;
-        q[t] = make_uint4(KEY_INF, KEY_INF, KEY_INF, KEY_INF);
-        if (v < NV && gb >= 0 && gb < (int64_t)nsorted) {
-            const size_t idx = srow + (size_t)gb * 64 + (v * E) % 64;
-            q[t] = F16 ? *(const uint4*)((const uint16_t*)sorted + idx)
-                       : *(const uint4*)((const uint32_t*)sorted + idx);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < VPT; t++) {
-        const int v = threadIdx.x + MB_BLOCKS * t;
-        if (v >= NV) break;
-        const uint32_t lb = (v * E) / 64, i0 = (v * E) % 64;
-        const int64_t gb = (int64_t)b0 - 2 + lb;
-        const uint32_t w[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            uint32_t key = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
-            if (!(gb >= 0 && gb < (int64_t)nsorted)) key = KEY_INF;
-            // block `nsorted` (m a multiple of 64) holds Nyquist alone
-            if (gb == (int64_t)nsorted && gb < (int64_t)nblocks && i0 + e == 0)
-                key = load_key(spec_t, F16, srow + m);
-            s[lb * STRIDE + i0 + e] = (KeyT)key;  // FP16: KEY_INF keeps its low 16 bits
-        }
-    }
-    __syncthreads();
-    const uint32_t b = b0 + threadIdx.x;
-    if (b >= nblocks) return;
+// Whitening scale of block b (lib.rs:45-60) from the sorted blocks b-2 .. b+2
+// at s + (lb + j) * STRIDE: the exact window median is found by a cut search --
+// a position p_j in every block with sum p_j = len/2 such that all keys left of
+// the cut are <= all keys right of it; the median is then the smallest key
+// right of the cut.  Blocks outside [0, nblocks) hold nothing and are not read.
+template <bool F16, typename KeyT, int STRIDE>
+__device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t nblocks, uint32_t nbins,
+                              float floor) {
     int nn[5];
     uint32_t L[5], U[5];
     int p[5];
@@ -264,7 +216,7 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
             cnt = s1 > s0 ? (int)(s1 - s0) : 0;
         }
         nn[j] = cnt;
-        A[j] = s + (threadIdx.x + j) * STRIDE;
+        A[j] = s + (size_t)(lb + j) * STRIDE;
         p[j] = cnt >> 1;
         len += cnt;
     }
@@ -331,19 +283,249 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
     float med = len > 0 ? key_power(umin - 1u, F16) : 0.0f;
     if (len == 0) med = 0.0f;  // rfi::median of an empty slice is 0.0
     const float LN2 = 0.693147180559945309417232121458176568f;
-    const float scale = fmaxf(fmaxf(med / LN2, floors[trial]), 1e-20f);
-    scales[(size_t)trial * nblocks + b] = scale;
+    return fmaxf(fmaxf(med / LN2, floor), 1e-20f);
+}
+
+// Stage 2: one thread per 64-bin block b; the five sorted blocks b-2 .. b+2 of
+// its window are staged in LDS.  Block `nsorted` (present when m is a multiple
+// of 64) holds Nyquist alone.
+template <bool F16>
+__global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
+    const void* __restrict__ sorted, const void* __restrict__ spec_t, uint32_t m, uint32_t nbins,
+    uint32_t nblocks, uint64_t spec_pitch, const float* __restrict__ floors,
+    float* __restrict__ scales) {
+    // FP16 keys are staged as 16-bit values (66 per block: an odd word stride),
+    // which halves the footprint and doubles the workgroups per CU
+    using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
+    constexpr int STRIDE = F16 ? 2 * 33 : (int)MB_STRIDE;
+    __shared__ KeyT s[(MB_BLOCKS + 4) * STRIDE];
+    constexpr int E = F16 ? 8 : 4;                                  // keys per 16-byte load
+    constexpr int NV = (int)((MB_BLOCKS + 4) * 64 / E);             // vectors staged
+    constexpr int VPT = (NV + (int)MB_BLOCKS - 1) / (int)MB_BLOCKS;  // per thread
+    const uint2 wb = xcd_block();
+    const uint32_t trial = wb.y;
+    const uint32_t b0 = wb.x * MB_BLOCKS;
+    const size_t srow = (size_t)trial * spec_pitch;
+    const uint32_t nsorted = (m + 63) / 64;
+    uint4 q[VPT];
+#pragma unroll
+    for (int t = 0; t < VPT; t++) {
+        const int v = threadIdx.x + MB_BLOCKS * t;
+        const int64_t gb = (int64_t)b0 - 2 + (v * E) / 64;
+        q[t] = make_uint4(KEY_INF, KEY_INF, KEY_INF, KEY_INF);
+        if (v < NV && gb >= 0 && gb < (int64_t)nsorted) {
+            const size_t idx = srow + (size_t)gb * 64 + (v * E) % 64;
+            q[t] = F16 ? *(const uint4*)((const uint16_t*)sorted + idx)
+                       : *(const uint4*)((const uint32_t*)sorted + idx);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < VPT; t++) {
+        const int v = threadIdx.x + MB_BLOCKS * t;
+        if (v >= NV) break;
+        const uint32_t lb = (v * E) / 64, i0 = (v * E) % 64;
+        const int64_t gb = (int64_t)b0 - 2 + lb;
+        const uint32_t w[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            uint32_t key = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
+            if (!(gb >= 0 && gb < (int64_t)nsorted)) key = KEY_INF;
+            // block `nsorted` (m a multiple of 64) holds Nyquist alone
+            if (gb == (int64_t)nsorted && gb < (int64_t)nblocks && i0 + e == 0)
+                key = load_key(spec_t, F16, srow + m);
+            s[lb * STRIDE + i0 + e] = (KeyT)key;  // FP16: KEY_INF keeps its low 16 bits
+        }
+    }
+    __syncthreads();
+    const uint32_t b = b0 + threadIdx.x;
+    if (b >= nblocks) return;
+    scales[(size_t)trial * nblocks + b] =
+        window_scale<F16, KeyT, STRIDE>(s, threadIdx.x, b, nblocks, nbins, floors[trial]);
+}
+
+
+// Fused whitening for the four-step layout (cw * m2 = SB_BLOCKS * 64): the
+// workgroup of columns [k1a, k1a + cw) writes its natural bins back in order,
+// sorts each of its SB_BLOCKS blocks in registers (one per lane), sorts the
+// two blocks on either side of its range (needed by its windows) across the
+// lanes of a wave, and finds every block's window median from LDS -- no
+// sorted copy goes through HBM.  The last workgroup of a trial also scores the
+// Nyquist block (m a multiple of 64).  Every workgroup recomputes the trial's
+// whitening floor (mean power * 1e-6, partial sums added in order).
+template <bool F16>
+constexpr int wf_stride() { return F16 ? 2 * 33 : (int)MB_STRIDE; }
+// The tile holds keys at their own width (FP16: 16 bits, which keeps four
+// workgroups per CU); element i of tile block B sits in 16-byte chunk
+// (i / CH) ^ h(B) of the block, CH keys per chunk.
+template <bool F16>
+__device__ __forceinline__ uint32_t wf_idx(uint32_t B, uint32_t i) {
+    constexpr uint32_t CH = F16 ? 8 : 4;
+    return B * 64 + ((((i / CH) ^ sb_h(B)) * CH) | (i % CH));
+}
+template <bool F16>
+constexpr size_t wf_lds_bytes() {
+    using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
+    const size_t tile = (size_t)SB_BLOCKS * 64 * sizeof(KeyT);
+    const size_t med = (size_t)(SB_BLOCKS + 6) * wf_stride<F16>() * sizeof(KeyT);
+    return (tile > med ? tile : med) + 4 * 64 * 4;  // + the four side blocks' keys
+}
+template <bool F16>
+__global__ __launch_bounds__(256) void k_whiten_fused(
+    const void* __restrict__ spec_t, uint32_t m, uint32_t m1, uint32_t m2, uint32_t cw,
+    uint32_t nbins, uint32_t nblocks, uint64_t spec_pitch, const double* __restrict__ partial,
+    uint32_t npart, void* __restrict__ raw_nat, float* __restrict__ scales) {
+    using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
+    constexpr int STRIDE = wf_stride<F16>();
+    constexpr size_t MAIN = wf_lds_bytes<F16>() - 4 * 64 * 4;
+    constexpr uint32_t CH = F16 ? 8 : 4;  // keys per 16-byte chunk
+    extern __shared__ __align__(16) uint32_t lds_w[];
+    KeyT* tile = (KeyT*)lds_w;
+    uint32_t* side = lds_w + MAIN / 4;  // [4][64]: blocks b0-2, b0-1, b0+SB, b0+SB+1
+    __shared__ float sfloor;
+    constexpr int E = F16 ? 8 : 4;
+    constexpr int VPT = (int)(SB_BLOCKS * 64 / E / 256);
+    const uint2 wb = xcd_block();
+    const uint32_t trial = wb.y;
+    const uint32_t k1a = wb.x * cw;
+    const uint32_t k_lo = k1a * m2, b0 = k_lo / 64;
+    const uint32_t nsorted = (m + 63) / 64;
+    const size_t srow = (size_t)trial * spec_pitch;
+    const uint32_t vpr = cw / E;
+    uint4 q[VPT];
+#pragma unroll
+    for (int t = 0; t < VPT; t++) {
+        const uint32_t v = threadIdx.x + 256u * t;
+        const uint32_t k2 = v / vpr, qq = v - k2 * vpr;
+        const size_t idx = srow + (size_t)k2 * m1 + k1a + qq * E;
+        q[t] = F16 ? *(const uint4*)((const uint16_t*)spec_t + idx) : *(const uint4*)((const uint32_t*)spec_t + idx);
+    }
+    {  // side blocks: thread t takes element t % 64 of side block t / 64
+        const uint32_t sb = threadIdx.x >> 6, i = threadIdx.x & 63;
+        const int64_t gb = sb < 2 ? (int64_t)b0 - 2 + sb : (int64_t)b0 + SB_BLOCKS + (sb - 2);
+        uint32_t key = KEY_INF;
+        if (gb >= 0 && gb < (int64_t)nsorted) {
+            const uint32_t K = (uint32_t)gb * 64 + i;
+            if (K > 0 && K < m) key = load_key(spec_t, F16, srow + (size_t)(K % m2) * m1 + K / m2);
+        } else if (gb == (int64_t)nsorted && gb < (int64_t)nblocks && i == 0) {
+            key = load_key(spec_t, F16, srow + m);  // block nsorted holds Nyquist alone
+        }
+        side[threadIdx.x] = key;
+    }
+    if (threadIdx.x == 0) {
+        double S = 0.0;
+        for (uint32_t i = 0; i < npart; i++) S += partial[(size_t)trial * npart + i];
+        sfloor = (float)(S / (double)nbins * 1e-6);
+    }
+#pragma unroll
+    for (int t = 0; t < VPT; t++) {
+        const uint32_t v = threadIdx.x + 256u * t;
+        const uint32_t k2 = v / vpr, qq = v - k2 * vpr;
+        const uint32_t w[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const uint32_t key = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
+            const uint32_t j = k2 + m2 * (qq * E + e);
+            tile[wf_idx<F16>(j >> 6, j & 63)] = (KeyT)key;
+        }
+    }
+    __syncthreads();
+    // natural-order raw spectrum, 8 keys per thread and step
+    for (uint32_t j = 8 * threadIdx.x; j < SB_BLOCKS * 64; j += 8 * blockDim.x) {
+        const uint32_t Bj = j >> 6, i = j & 63;
+        uint32_t k[8];
+        if (F16) {
+            const uint4 a = *(const uint4*)(tile + wf_idx<F16>(Bj, i));
+            const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int e = 0; e < 8; e++) k[e] = (w[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+        } else {
+            const uint4 a = *(const uint4*)(tile + wf_idx<F16>(Bj, i));
+            const uint4 b = *(const uint4*)(tile + wf_idx<F16>(Bj, i + 4));
+            k[0] = a.x, k[1] = a.y, k[2] = a.z, k[3] = a.w, k[4] = b.x, k[5] = b.y, k[6] = b.z, k[7] = b.w;
+        }
+        store_keys8<F16>(raw_nat, srow + k_lo + j, k);
+    }
+    if (k1a + cw == m1 && threadIdx.x == 0) {
+        const uint32_t key = load_key(spec_t, F16, srow + m);  // Nyquist, outside every tile
+        if (F16)
+            ((uint16_t*)raw_nat)[srow + m] = (uint16_t)key;
+        else
+            ((uint32_t*)raw_nat)[srow + m] = key;
+    }
+    const uint32_t B = threadIdx.x;
+    uint32_t v[64];
+#pragma unroll
+    for (int c = 0; c < 64 / (int)CH; c++) {
+        const uint4 qv = *(const uint4*)(tile + wf_idx<F16>(B, c * CH));
+        const uint32_t w[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+        for (int e = 0; e < (int)CH; e++) v[c * CH + e] = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
+    }
+    if (k_lo == 0 && B == 0) v[0] = KEY_INF;  // DC is not part of any window
+    bitonic_sort_regs<64>(v);
+    // side block w (waves 0..3): bitonic network across the wave's lanes
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = side[threadIdx.x];
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t pv = (uint32_t)__shfl_xor((int)x, j);
+            const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+            x = lower == up ? min(x, pv) : max(x, pv);
+        }
+    __syncthreads();  // every read of the tile is done
+    // sorted blocks in the median layout: local block lb = global block - b0 + 2
+    KeyT* sk = (KeyT*)lds_w;
+#pragma unroll
+    for (int i = 0; i < 64; i++) sk[(B + 2) * STRIDE + i] = (KeyT)v[i];  // FP16: KEY_INF keeps its low 16 bits
+    sk[(wave < 2 ? wave : SB_BLOCKS + wave) * STRIDE + lane] = (KeyT)x;
+    if (threadIdx.x < 2 * STRIDE)  // local blocks SB + 4 and SB + 5: never counted, kept defined
+        sk[(SB_BLOCKS + 4) * STRIDE + threadIdx.x] = (KeyT)KEY_INF;
+    __syncthreads();
+    const float fl = sfloor;
+    scales[(size_t)trial * nblocks + b0 + B] = window_scale<F16, KeyT, STRIDE>(sk, B, b0 + B, nblocks, nbins, fl);
+    if (B == 0 && k1a + cw == m1 && nsorted < nblocks)  // the Nyquist block
+        scales[(size_t)trial * nblocks + nsorted] =
+            window_scale<F16, KeyT, STRIDE>(sk, SB_BLOCKS, nsorted, nblocks, nbins, fl);
+}
+
+static uint32_t sort_columns(const SpecGeom& g) {  // columns per sorting workgroup
+    uint32_t cw = g.m1;
+    while (cw > 1 && (uint64_t)cw * g.m2 > (uint64_t)SB_BLOCKS * 64) cw >>= 1;
+    return cw;
+}
+bool whiten_fused(const SpecGeom& g) {
+    const uint32_t cw = sort_columns(g);
+    return g.m1 > 1 && (uint64_t)cw * g.m2 == (uint64_t)SB_BLOCKS * 64 && cw % 8 == 0;
 }
 
 void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp16,
                    uint32_t ntrials, const double* partial, uint32_t npartials, void* raw_nat,
                    void* sorted, float* floors, float* scales) {
     // columns per sorting workgroup: up to SB_BLOCKS natural blocks
-    uint32_t cw = g.m1;
-    while (cw > 1 && (uint64_t)cw * g.m2 > (uint64_t)SB_BLOCKS * 64) cw >>= 1;
+    const uint32_t cw = sort_columns(g);
     const uint32_t nbk = (cw * g.m2 + 63) / 64;
     const size_t shm = (size_t)nbk * 64 * sizeof(uint32_t);
     const dim3 gs(g.m1 / cw, ntrials), gm((g.nblocks + MB_BLOCKS - 1) / MB_BLOCKS, ntrials);
+    if (whiten_fused(g)) {
+        if (fp16) {
+            constexpr size_t shm = wf_lds_bytes<true>();
+            if (shm > 64 * 1024)
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_whiten_fused<true>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            hipLaunchKernelGGL(k_whiten_fused<true>, gs, dim3(256), shm, s, spec_t, g.m, g.m1, g.m2, cw, g.nbins,
+                               g.nblocks, g.spec_pitch, partial, npartials, raw_nat, scales);
+        } else {
+            constexpr size_t shm = wf_lds_bytes<false>();
+            if (shm > 64 * 1024)
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_whiten_fused<false>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            hipLaunchKernelGGL(k_whiten_fused<false>, gs, dim3(256), shm, s, spec_t, g.m, g.m1, g.m2, cw,
+                               g.nbins, g.nblocks, g.spec_pitch, partial, npartials, raw_nat, scales);
+        }
+        return;
+    }
     if (fp16) {
         hipLaunchKernelGGL(k_sort_blocks<true>, gs, dim3(256), shm, s, spec_t, g.m, g.m1, g.m2, cw,
                            g.nbins, g.spec_pitch, partial, npartials, raw_nat, sorted, floors);
