@@ -344,30 +344,24 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
 }
 
 
-// Fused whitening for the four-step layout (cw * m2 = SB_BLOCKS * 64): the
-// workgroup of columns [k1a, k1a + cw) writes its natural bins back in order,
-// sorts each of its SB_BLOCKS blocks in registers (one per lane), sorts the
-// two blocks on either side of its range (needed by its windows) across the
-// lanes of a wave, and finds every block's window median from LDS -- no
-// sorted copy goes through HBM.  The last workgroup of a trial also scores the
-// Nyquist block (m a multiple of 64).  Every workgroup recomputes the trial's
-// whitening floor (mean power * 1e-6, partial sums added in order).
+// Fused whitening for the four-step layout (cw * m2 = SB_BLOCKS * 64, m2 a
+// multiple of 64): the workgroup of columns [k1a, k1a + cw) owns the natural
+// blocks of those columns.  A natural block is 64 consecutive rows of one
+// column of the [k2][k1] spectrum, so lane l of a wave loads the block of
+// column k1a + l % cw straight into registers (each load instruction reads
+// whole row segments), writes it back in natural order, and sorts it; the two
+// blocks on either side of the range (needed by its windows) are sorted across
+// the lanes of a wave; every block's window median is then found from LDS --
+// no transpose through LDS and no sorted copy through HBM.  The last workgroup
+// of a trial also scores the Nyquist block (m a multiple of 64).  Every
+// workgroup recomputes the trial's whitening floor (mean power * 1e-6, partial
+// sums added in order).
 template <bool F16>
 constexpr int wf_stride() { return F16 ? 2 * 33 : (int)MB_STRIDE; }
-// The tile holds keys at their own width (FP16: 16 bits, which keeps four
-// workgroups per CU); element i of tile block B sits in 16-byte chunk
-// (i / CH) ^ h(B) of the block, CH keys per chunk.
-template <bool F16>
-__device__ __forceinline__ uint32_t wf_idx(uint32_t B, uint32_t i) {
-    constexpr uint32_t CH = F16 ? 8 : 4;
-    return B * 64 + ((((i / CH) ^ sb_h(B)) * CH) | (i % CH));
-}
 template <bool F16>
 constexpr size_t wf_lds_bytes() {
     using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
-    const size_t tile = (size_t)SB_BLOCKS * 64 * sizeof(KeyT);
-    const size_t med = (size_t)(SB_BLOCKS + 6) * wf_stride<F16>() * sizeof(KeyT);
-    return (tile > med ? tile : med) + 4 * 64 * 4;  // + the four side blocks' keys
+    return (size_t)(SB_BLOCKS + 6) * wf_stride<F16>() * sizeof(KeyT) + 4 * 64 * 4;  // + side blocks
 }
 template <bool F16>
 __global__ __launch_bounds__(256) void k_whiten_fused(
@@ -377,27 +371,25 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
     using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
     constexpr int STRIDE = wf_stride<F16>();
     constexpr size_t MAIN = wf_lds_bytes<F16>() - 4 * 64 * 4;
-    constexpr uint32_t CH = F16 ? 8 : 4;  // keys per 16-byte chunk
     extern __shared__ __align__(16) uint32_t lds_w[];
-    KeyT* tile = (KeyT*)lds_w;
+    KeyT* sk = (KeyT*)lds_w;             // sorted blocks, local block lb = global - b0 + 2
     uint32_t* side = lds_w + MAIN / 4;  // [4][64]: blocks b0-2, b0-1, b0+SB, b0+SB+1
     __shared__ float sfloor;
-    constexpr int E = F16 ? 8 : 4;
-    constexpr int VPT = (int)(SB_BLOCKS * 64 / E / 256);
     const uint2 wb = xcd_block();
     const uint32_t trial = wb.y;
     const uint32_t k1a = wb.x * cw;
     const uint32_t k_lo = k1a * m2, b0 = k_lo / 64;
     const uint32_t nsorted = (m + 63) / 64;
+    const uint32_t cpb = m2 / 64;  // blocks per column
     const size_t srow = (size_t)trial * spec_pitch;
-    const uint32_t vpr = cw / E;
-    uint4 q[VPT];
+    // this lane's block: column k1a + col, rows 64 ch .. 64 ch + 63
+    const uint32_t col = threadIdx.x % cw, ch = threadIdx.x / cw;
+    const uint32_t B = ch + cpb * col;  // local natural block
+    uint32_t v[64];
+    {
+        const size_t base = srow + (size_t)(64 * ch) * m1 + k1a + col;
 #pragma unroll
-    for (int t = 0; t < VPT; t++) {
-        const uint32_t v = threadIdx.x + 256u * t;
-        const uint32_t k2 = v / vpr, qq = v - k2 * vpr;
-        const size_t idx = srow + (size_t)k2 * m1 + k1a + qq * E;
-        q[t] = F16 ? *(const uint4*)((const uint16_t*)spec_t + idx) : *(const uint4*)((const uint32_t*)spec_t + idx);
+        for (int r = 0; r < 64; r++) v[r] = load_key(spec_t, F16, base + (size_t)r * m1);
     }
     {  // side blocks: thread t takes element t % 64 of side block t / 64
         const uint32_t sb = threadIdx.x >> 6, i = threadIdx.x & 63;
@@ -416,53 +408,24 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
         for (uint32_t i = 0; i < npart; i++) S += partial[(size_t)trial * npart + i];
         sfloor = (float)(S / (double)nbins * 1e-6);
     }
+    // natural-order raw spectrum: the block's 64 keys
+    {
+        const size_t o = srow + (size_t)(b0 + B) * 64;
 #pragma unroll
-    for (int t = 0; t < VPT; t++) {
-        const uint32_t v = threadIdx.x + 256u * t;
-        const uint32_t k2 = v / vpr, qq = v - k2 * vpr;
-        const uint32_t w[4] = {q[t].x, q[t].y, q[t].z, q[t].w};
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            const uint32_t key = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
-            const uint32_t j = k2 + m2 * (qq * E + e);
-            tile[wf_idx<F16>(j >> 6, j & 63)] = (KeyT)key;
+        for (int c = 0; c < 8; c++) store_keys8<F16>(raw_nat, o + 8 * c, v + 8 * c);
+        if (k1a + cw == m1 && threadIdx.x == 0) {
+            const uint32_t key = load_key(spec_t, F16, srow + m);  // Nyquist, outside every block
+            if (F16)
+                ((uint16_t*)raw_nat)[srow + m] = (uint16_t)key;
+            else
+                ((uint32_t*)raw_nat)[srow + m] = key;
         }
-    }
-    __syncthreads();
-    // natural-order raw spectrum, 8 keys per thread and step
-    for (uint32_t j = 8 * threadIdx.x; j < SB_BLOCKS * 64; j += 8 * blockDim.x) {
-        const uint32_t Bj = j >> 6, i = j & 63;
-        uint32_t k[8];
-        if (F16) {
-            const uint4 a = *(const uint4*)(tile + wf_idx<F16>(Bj, i));
-            const uint32_t w[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-            for (int e = 0; e < 8; e++) k[e] = (w[e >> 1] >> (16 * (e & 1))) & 0xffffu;
-        } else {
-            const uint4 a = *(const uint4*)(tile + wf_idx<F16>(Bj, i));
-            const uint4 b = *(const uint4*)(tile + wf_idx<F16>(Bj, i + 4));
-            k[0] = a.x, k[1] = a.y, k[2] = a.z, k[3] = a.w, k[4] = b.x, k[5] = b.y, k[6] = b.z, k[7] = b.w;
-        }
-        store_keys8<F16>(raw_nat, srow + k_lo + j, k);
-    }
-    if (k1a + cw == m1 && threadIdx.x == 0) {
-        const uint32_t key = load_key(spec_t, F16, srow + m);  // Nyquist, outside every tile
-        if (F16)
-            ((uint16_t*)raw_nat)[srow + m] = (uint16_t)key;
-        else
-            ((uint32_t*)raw_nat)[srow + m] = key;
-    }
-    const uint32_t B = threadIdx.x;
-    uint32_t v[64];
-#pragma unroll
-    for (int c = 0; c < 64 / (int)CH; c++) {
-        const uint4 qv = *(const uint4*)(tile + wf_idx<F16>(B, c * CH));
-        const uint32_t w[4] = {qv.x, qv.y, qv.z, qv.w};
-#pragma unroll
-        for (int e = 0; e < (int)CH; e++) v[c * CH + e] = F16 ? (w[e >> 1] >> (16 * (e & 1))) & 0xffffu : w[e];
     }
     if (k_lo == 0 && B == 0) v[0] = KEY_INF;  // DC is not part of any window
     bitonic_sort_regs<64>(v);
+#pragma unroll
+    for (int i = 0; i < 64; i++) sk[(B + 2) * STRIDE + i] = (KeyT)v[i];  // FP16: KEY_INF keeps its low 16 bits
+    __syncthreads();  // side keys and the floor are in LDS
     // side block w (waves 0..3): bitonic network across the wave's lanes
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = side[threadIdx.x];
@@ -474,18 +437,14 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
             const bool up = (lane & k) == 0, lower = (lane & j) == 0;
             x = lower == up ? min(x, pv) : max(x, pv);
         }
-    __syncthreads();  // every read of the tile is done
-    // sorted blocks in the median layout: local block lb = global block - b0 + 2
-    KeyT* sk = (KeyT*)lds_w;
-#pragma unroll
-    for (int i = 0; i < 64; i++) sk[(B + 2) * STRIDE + i] = (KeyT)v[i];  // FP16: KEY_INF keeps its low 16 bits
     sk[(wave < 2 ? wave : SB_BLOCKS + wave) * STRIDE + lane] = (KeyT)x;
     if (threadIdx.x < 2 * STRIDE)  // local blocks SB + 4 and SB + 5: never counted, kept defined
         sk[(SB_BLOCKS + 4) * STRIDE + threadIdx.x] = (KeyT)KEY_INF;
     __syncthreads();
     const float fl = sfloor;
-    scales[(size_t)trial * nblocks + b0 + B] = window_scale<F16, KeyT, STRIDE>(sk, B, b0 + B, nblocks, nbins, fl);
-    if (B == 0 && k1a + cw == m1 && nsorted < nblocks)  // the Nyquist block
+    const uint32_t bm = threadIdx.x;  // medians in natural block order
+    scales[(size_t)trial * nblocks + b0 + bm] = window_scale<F16, KeyT, STRIDE>(sk, bm, b0 + bm, nblocks, nbins, fl);
+    if (bm == 0 && k1a + cw == m1 && nsorted < nblocks)  // the Nyquist block
         scales[(size_t)trial * nblocks + nsorted] =
             window_scale<F16, KeyT, STRIDE>(sk, SB_BLOCKS, nsorted, nblocks, nbins, fl);
 }
@@ -497,7 +456,7 @@ static uint32_t sort_columns(const SpecGeom& g) {  // columns per sorting workgr
 }
 bool whiten_fused(const SpecGeom& g) {
     const uint32_t cw = sort_columns(g);
-    return g.m1 > 1 && (uint64_t)cw * g.m2 == (uint64_t)SB_BLOCKS * 64 && cw % 8 == 0;
+    return g.m1 > 1 && (uint64_t)cw * g.m2 == (uint64_t)SB_BLOCKS * 64 && g.m2 % 64 == 0;
 }
 
 void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp16,
