@@ -141,26 +141,39 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // ------------------------------------------------- integer dedispersion (u8)
 // Workgroup = 8 waves sharing one time tile of DD_TILE = 512 outputs and a block
 // of DD_TRIALS = 64 consecutive (ascending) DM trials; each wave owns DD_M = 8
-// trials.  A channel's samples are staged into LDS as 32-bit words
-//     w[i] = s[base + i] | s[base + i + 256] << 16,
+// trials, and lane l owns outputs t0 + l + 64 k (k < 8).  A channel's samples
+// are staged into LDS as 16-byte words of eight u16 lanes,
+//     w[i] = (s[base + i + 64 k], k < 8)      (dword j: k = 2j low, 2j + 1 high),
 // with base = the block's smallest delay of the channel rounded down to 4.  A
-// trial whose delay is base + e needs words e .. e + 255; lane l takes words
-// e + 4l .. e + 4l + 3 with one 16-byte ds_read_b128.  ds_read_b128 needs a
-// 16-byte aligned address, so every row is staged four times, copy k starting
-// k words later; the trial reads copy e & 3 at word e & ~3.  The 64 lanes of a
-// wave read 1 KiB of consecutive LDS, free of bank conflicts.  A row is only as
-// long as the delays of the trials reading it require.
+// trial whose delay is base + e reads word e + l in lane l with one aligned
+// ds_read_b128: the 64 lanes read 1 KiB of consecutive LDS, free of bank
+// conflicts, and a row is only as long as the delays of the trials reading it
+// require (their e range + 64 words).
 //
-// Rows are added in pairs with v_add3_u32 on the u16 pairs: within a "phase"
-// of at most 65535 / (2 maxval) rows no half can carry, so the 32-bit add is
+// Channels are summed in pairs (c, c + 1).  Within a block the relative delay
+// delta = delay[c + 1] - delay[c] takes few distinct values (two or three when
+// the block spans a small DM range), so for each value the workgroup builds one
+// "pair row" P[i] = w_c[i] + w_(c+1)[i + delta - (base_(c+1) - base_c)] (u16
+// lanes of at most 2 * 255) and every trial reads one pair row per channel
+// pair: half the LDS reads and half the adds of reading both channel rows.
+// Pairs whose delta takes more than kmax values in a block are summed as two
+// single-channel rows.
+//
+// Rows are added in pairs with v_add3_u32 on the u16 lanes: within a "phase"
+// of at most 65535 / (2 maxval) rows no lane can carry, so the 32-bit add is
 // two exact 16-bit adds.  At each phase end the halves are added into running
 // sums mod 2^16 whose carries are counted in 4-bit fields, so totals up to 2^20
 // are exact.
 //
-// Staging is pipelined: while a stage is summed, the raw bytes of the next
-// stage's channels (one channel per wave) and the record of the stage after it
-// are loaded straight into LDS (MUBUF loads with the LDS bit, no registers
-// held); at the stage boundary the waves turn the raw bytes into the rows.
+// A planner kernel lays out each block as stages: consecutive slots (a slot is
+// a channel pair or a single channel) whose rows fit the LDS row area and whose
+// raw windows (one per channel) fit the raw area.  A stage is one fixed-size
+// record -- header, row records and every trial's 16-bit LDS read address per
+// slot.  Staging is pipelined: while a stage is summed, the raw windows of the
+// next stage and the record of the stage after it are loaded straight into LDS
+// (MUBUF loads with the LDS bit, no registers held, and no scalar loads, whose
+// out-of-order returns would force full LDS waits in the sums); at the stage
+// boundary the waves turn the raw bytes into the rows.
 //
 // Workgroups are numbered XCD-aware: consecutive workgroup ids go to different
 // XCDs, so id is remapped so that each XCD walks a contiguous run of
@@ -174,49 +187,32 @@ constexpr int DD_TRIALS = DD_WAVES * DD_M;
 static_assert(DD_TRIALS == (int)DEDISP_BLOCK_TRIALS, "block size shared with the host");
 constexpr int DD_XCDS = 8;
 constexpr int DD_AHEAD = 4;  // trials whose LDS reads are issued ahead of their adds
-constexpr int DD_ROW_BYTES = 56 * 1024;  // row copies of one stage (at most)
-constexpr int DD_RAW_BYTES = 8192;       // raw byte windows of one stage (at most)
+constexpr int DD_ROW_BYTES = 44 * 1024;  // rows of one stage (at most)
+constexpr int DD_RAW_BYTES = 16 * 1024;  // raw byte windows of one stage (at most)
 constexpr int DD_RSRC_WORD3 = 0x00020000;  // buffer descriptor word 3: raw 32-bit data
-
-// pair word of byte i (0..3) of windows a (low half) and b (high half)
-__device__ __forceinline__ uint32_t pair_word(uint32_t b, uint32_t a, int i) {
-    return __builtin_amdgcn_perm(b, a, 0x0c040c00u + 0x00010001u * (uint32_t)i);
-}
-
-// Channels are summed in pairs (c, c + 1).  Within a block of DD_TRIALS
-// trials the relative delay delta = delay[c + 1] - delay[c] takes few distinct
-// values (two or three when the block spans a small DM range), so for each
-// value the workgroup builds one "pair row" of words
-//     P[i] = w_c[i] + w_(c+1)[i + delta - (base_(c+1) - base_c)]
-// (u16 halves of at most 2 * 255) and every trial reads one pair row per
-// channel pair: half the LDS reads and half the adds of reading both channel
-// rows.  Pairs whose delta takes more than kmax values in a block are summed as
-// two single-channel rows.
-//
-// A planner kernel lays out each block as stages: consecutive slots (a slot is
-// a channel pair or a single channel) whose rows fit the LDS row area and
-// whose raw windows (one per channel) fit the raw area, one window per wave.
-// A stage is one fixed-size record -- header, row records and every trial's
-// 16-bit LDS read address per slot -- that the dedispersion kernel copies
-// into LDS two stages ahead with direct global->LDS loads, so the stage loop
-// issues no scalar loads (whose out-of-order returns would force full LDS
-// waits in the sums).
-constexpr int DDP_MAX_ROWS = 2 * DD_WAVES;  // wave w builds rows w and w + DD_WAVES
-constexpr int DDP_MAX_SLOTS = DD_WAVES;     // slots per stage
-constexpr int DDP_KMAX = 3;                 // pair rows per channel pair (at most)
-// stage record (u32 words): [0] nwin | nslot << 8 | nrow << 16, [4..11] window
-// channels, [16 + 4 r ..] row r: rowoff | quads << 16, raw byte of word L of
-// channel c in its window, the same for channel c + 1, cwin | c1win << 8 |
-// paired << 16; [80 + 32 s ..] read addresses of slot s (DD_TRIALS x u16)
-constexpr int DDP_ROW_REC = 16;
+constexpr int DDP_MAX_WINDOWS = 2 * DD_WAVES;  // wave w loads windows w and w + DD_WAVES
+constexpr int DDP_MAX_ROWS = 3 * DD_WAVES;     // wave w builds rows w, w + 8, w + 16
+constexpr int DDP_MAX_SLOTS = 16;              // slots per stage
+constexpr int DDP_KMAX = 3;                    // pair rows per channel pair (at most)
+// stage record (u32 words): [0] nwin | nslot << 8 | nrow << 16, [4 ..] window
+// channels, [DDP_ROW_REC + 4 r ..] row r: rowoff | quads << 16, raw byte of
+// word L of channel c in its window, the same for channel c + 1, cwin | c1win
+// << 8 | paired << 16; [DDP_LAD_REC + 32 s ..] read addresses of slot s
+// (DD_TRIALS x u16)
+constexpr int DDP_ROW_REC = 4 + DDP_MAX_WINDOWS;
 constexpr int DDP_LAD_REC = DDP_ROW_REC + 4 * DDP_MAX_ROWS;
-constexpr int DDP_STAGE_WORDS = DDP_LAD_REC + DDP_MAX_SLOTS * DD_TRIALS / 2;  // 336
-constexpr int DDP_STAGE_CHUNKS = (DDP_STAGE_WORDS + 63) / 64;                   // 64-word loads
-constexpr int DDP_STAGE_PITCH = 64 * DDP_STAGE_CHUNKS;                          // 384 words
+constexpr int DDP_STAGE_WORDS = DDP_LAD_REC + DDP_MAX_SLOTS * DD_TRIALS / 2;
+constexpr int DDP_STAGE_CHUNKS = (DDP_STAGE_WORDS + 63) / 64;  // 64-word loads
+constexpr int DDP_STAGE_PITCH = 64 * DDP_STAGE_CHUNKS;
 constexpr int DDP_PLAN_HEAD = 64;  // word 0: stage count
 
 static size_t ddp_plan_words(uint32_t nchans) {  // per block; stages <= slots <= nchans
     return DDP_PLAN_HEAD + (size_t)nchans * DDP_STAGE_PITCH;
+}
+
+// u16 pair of byte i (0..3) of dwords a (low lane) and b (high lane)
+__device__ __forceinline__ uint32_t pair_word(uint32_t b, uint32_t a, uint32_t i) {
+    return __builtin_amdgcn_perm(b, a, 0x0c040c00u + 0x00010001u * i);
 }
 
 // Raw window of channel ch in a tile starting at t0: bytes from
@@ -245,6 +241,12 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
     uint32_t* pb = plan + (size_t)blk * plan_words;
     auto delay = [&](uint32_t m, uint32_t c) {
         return delays[(size_t)min(d0 + m, ntrials - 1) * nchans + c];
+    };
+    // row of words [L, L + W) read by trials with e in [lo, hi]: L = lo & ~3 and
+    // W = hi - L + 64 rounded up to 4 (a trial reads words e .. e + 63)
+    auto row_lw = [](uint32_t lo, uint32_t hi) {
+        const uint32_t L = lo & ~3u;
+        return L | ((hi - L + DD_TILE / 8 + 3) & ~3u) << 16;
     };
     for (uint32_t c = threadIdx.x; c < nchans; c += blockDim.x) {
         const int32_t b = delay(0, c) & ~3;
@@ -278,15 +280,18 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
         }
         ukind[u] = K;
         for (uint32_t k = 0; k < K; k++) {
-            const uint32_t L = (uint32_t)lo[k] & ~3u, W = ((uint32_t)hi[k] & ~3u) - L + DD_TILE / 2;
             udel[u * DDP_KMAX + k] = del[k];
-            uLW[u * DDP_KMAX + k] = L | W << 16;
+            uLW[u * DDP_KMAX + k] = row_lw((uint32_t)lo[k], (uint32_t)hi[k]);
         }
-        if (K == 0)  // single-channel rows of c (and c + 1): L = 0
+        if (K == 0)  // single-channel rows of c (and c + 1)
             for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
-                int32_t emax = 0;
-                for (uint32_t m = 0; m < (uint32_t)DD_TRIALS; m++) emax = max(emax, delay(m, c + j) - sbase[c + j]);
-                uLW[u * DDP_KMAX + j] = (((uint32_t)emax & ~3u) + DD_TILE / 2) << 16;
+                int32_t emin = 1 << 30, emax = 0;
+                for (uint32_t m = 0; m < (uint32_t)DD_TRIALS; m++) {
+                    const int32_t e = delay(m, c + j) - sbase[c + j];
+                    emin = min(emin, e);
+                    emax = max(emax, e);
+                }
+                uLW[u * DDP_KMAX + j] = row_lw((uint32_t)emin, (uint32_t)emax);
             }
     }
     __syncthreads();
@@ -318,14 +323,16 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
             nslot++;
             nslot_all++;
         };
+        auto fits = [&](uint32_t w, uint32_t r, uint32_t b) {
+            return nwin + w <= max_windows && nrow + r <= (uint32_t)DDP_MAX_ROWS && bytes + b <= row_budget &&
+                   nslot < (uint32_t)DDP_MAX_SLOTS;
+        };
         for (uint32_t u = 0; u < npair; u++) {
             const uint32_t c = 2 * u, K = ukind[u];
             if (K) {
                 uint32_t need = 0;
                 for (uint32_t k = 0; k < K; k++) need += 16 * (uLW[u * DDP_KMAX + k] >> 16);
-                if (nwin + 2 > max_windows || nrow + K > (uint32_t)DDP_MAX_ROWS ||
-                    bytes + need > row_budget || nslot == (uint32_t)DDP_MAX_SLOTS)
-                    close();
+                if (!fits(2, K, need)) close();
                 uslot[u] = nslot_all;
                 st[4 + nwin] = c;
                 st[5 + nwin] = c + 1;
@@ -339,13 +346,11 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
                 put_slot();
             } else {
                 for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
-                    const uint32_t W = uLW[u * DDP_KMAX + j] >> 16;
-                    if (nwin + 1 > max_windows || nrow + 1 > (uint32_t)DDP_MAX_ROWS ||
-                        bytes + 16 * W > row_budget || nslot == (uint32_t)DDP_MAX_SLOTS)
-                        close();
+                    const uint32_t L = uLW[u * DDP_KMAX + j] & 0xffffu, W = uLW[u * DDP_KMAX + j] >> 16;
+                    if (!fits(1, 1, 16 * W)) close();
                     if (j == 0) uslot[u] = nslot_all;
                     st[4 + nwin] = c + j;
-                    put_row(u * DDP_KMAX + j, W, ddp_mis(sbase[c + j]), 0, nwin, 0, 0);
+                    put_row(u * DDP_KMAX + j, W, ddp_mis(sbase[c + j]) + L, 0, nwin, 0, 0);
                     nwin++;
                     put_slot();
                 }
@@ -368,13 +373,13 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
             const uint32_t e = (uint32_t)(dc - sbase[c]);
             uint32_t k = 0;
             while (k + 1 < K && udel[u * DDP_KMAX + k] != dl) k++;
-            const uint32_t L = uLW[u * DDP_KMAX + k] & 0xffffu, W = uLW[u * DDP_KMAX + k] >> 16;
-            lad(uslot[u])[m] = (uint16_t)(uRO[u * DDP_KMAX + k] + ((e & 3) * W + (e & ~3u) - L) * 4);
+            const uint32_t L = uLW[u * DDP_KMAX + k] & 0xffffu;
+            lad(uslot[u])[m] = (uint16_t)(uRO[u * DDP_KMAX + k] + (e - L) * 16);
         } else {
             for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
                 const uint32_t e = (uint32_t)(delay(m, c + j) - sbase[c + j]);
-                const uint32_t W = uLW[u * DDP_KMAX + j] >> 16;
-                lad(uslot[u] + j)[m] = (uint16_t)(uRO[u * DDP_KMAX + j] + ((e & 3) * W + (e & ~3u)) * 4);
+                const uint32_t L = uLW[u * DDP_KMAX + j] & 0xffffu;
+                lad(uslot[u] + j)[m] = (uint16_t)(uRO[u * DDP_KMAX + j] + (e - L) * 16);
             }
         }
     }
@@ -405,10 +410,13 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     const uint32_t t0 = tile * DD_TILE;
     const uint32_t d0 = blk * DD_TRIALS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
     const uint32_t* pb = plan + (size_t)blk * plan_words;
     const uint32_t nstages = __builtin_amdgcn_readfirstlane(pb[0]);
 
+    // acc16: this phase's sums (u16 lanes, none can overflow within a phase);
+    // lo: running sums mod 2^16 per lane; cy: carries out of each lane, 4 bits
+    // per output (bits 4j + 16h for dword j, half h; the host keeps every total
+    // below 2^20)
     uint32_t acc16[DD_M][4], lo[DD_M][4], cy[DD_M];
 #pragma unroll
     for (int m = 0; m < DD_M; m++) {
@@ -430,37 +438,39 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
             }
     };
     uint32_t in_phase = 0;
-    const char* lds_lane = (const char*)lds + 16 * lane;
+    const char* lds_lane = (const char*)lds + 16 * (threadIdx.x & 63);
     for (uint32_t i = threadIdx.x; i < nchans; i += DD_THREADS) lbase[i] = bases[(size_t)blk * nchans + i];
-    // record of stage st into srec[st % 3]: wave w < DDP_STAGE_CHUNKS copies 64 words
+    // record of stage st into srec[st % 3], 64 words per wave and load
     auto load_record = [&](uint32_t st) {
-        if ((uint32_t)wave >= (uint32_t)DDP_STAGE_CHUNKS) return;
         uint32_t ln = threadIdx.x;  // recomputed per call: hoisted per-lane values would spill
         asm volatile("" : "+v"(ln));
         ln &= 63;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             (void*)(pb + DDP_PLAN_HEAD + (size_t)st * DDP_STAGE_PITCH), 0, 4 * DDP_STAGE_PITCH, DD_RSRC_WORD3);
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, srec[st % 3] + 64 * wave, 4, 4 * (64 * wave + ln), 0, 0, 0);
+        for (int ck = wave; ck < DDP_STAGE_CHUNKS; ck += DD_WAVES)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, srec[st % 3] + 64 * ck, 4, 4 * (64 * ck + ln), 0, 0, 0);
     };
-    // raw windows of stage st: wave w < nwin loads channel w's window (lanes
+    // raw windows of stage st: wave w loads windows w and w + DD_WAVES (lanes
     // whose 16 bytes fall outside the row read its nearest 16 in-row bytes
     // instead: those samples only reach outputs at or past n, or words no
     // trial reads)
     auto load_windows = [&](uint32_t st) {
         const uint32_t* sr = srec[st % 3];
         const uint32_t nwin = __builtin_amdgcn_readfirstlane(sr[0]) & 0xffu;
-        if ((uint32_t)wave >= nwin) return;
         uint32_t ln = threadIdx.x;
         asm volatile("" : "+v"(ln));
         ln &= 63;
-        const uint32_t c = __builtin_amdgcn_readfirstlane(sr[4 + wave]);
-        const int64_t start = (int64_t)t0 + (int64_t)(lbase[c] & ~15) - 16;
-        const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(data + (size_t)c * pitch), 0, (int)pitch, DD_RSRC_WORD3);
-        for (uint32_t k = 0; k < RB; k += 1024) {
-            int64_t p = start + (int64_t)k + 16 * (int64_t)ln;
-            p = p < 0 ? 0 : (p > (int64_t)pitch - 16 ? (int64_t)pitch - 16 : p);
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, raw + (size_t)wave * RB + k, 16, (uint32_t)p, 0, 0, 0);
+        for (uint32_t wi = (uint32_t)wave; wi < nwin; wi += DD_WAVES) {
+            const uint32_t c = __builtin_amdgcn_readfirstlane(sr[4 + wi]);
+            // 32-bit offsets: t0 + base < pitch < 2^31
+            const int32_t start = (int32_t)t0 + (lbase[c] & ~15) - 16;
+            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(data + (size_t)c * pitch), 0, (int)pitch, DD_RSRC_WORD3);
+            for (uint32_t k = 0; k < RB; k += 1024) {
+                int32_t p = start + (int32_t)(k + 16 * ln);
+                p = p < 0 ? 0 : (p > (int32_t)pitch - 16 ? (int32_t)pitch - 16 : p);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, raw + (size_t)wi * RB + k, 16, (uint32_t)p, 0, 0, 0);
+            }
         }
     };
     load_record(0);
@@ -477,7 +487,10 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
         // sums are done with the rows
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         __syncthreads();
-        // rows: wave w builds rows w and w + DD_WAVES of the stage
+        // rows: wave w builds rows w, w + 8, w + 16.  Lane g builds words
+        // L + 4g .. L + 4g + 3 from dword g + 16 k (k < 8) of the windows; its
+        // four 16-byte stores go in a lane-rotated order, which spreads every
+        // 16 lanes' stores over all LDS banks.
         for (uint32_t rr = (uint32_t)wave; rr < nrow; rr += DD_WAVES) {
             uint32_t ln = threadIdx.x;  // as in load_record
             asm volatile("" : "+v"(ln));
@@ -487,52 +500,50 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
             const uint32_t offc = __builtin_amdgcn_readfirstlane(rec[1]);
             const uint32_t offc1 = __builtin_amdgcn_readfirstlane(rec[2]);
             const uint32_t r3 = __builtin_amdgcn_readfirstlane(rec[3]);
-            const uint32_t quads = r0 >> 16, W = 4 * quads;
+            const uint32_t quads = r0 >> 16;
             uint32_t* row = lds + (r0 & 0xffffu) / 4;
             const uint32_t* rc = (const uint32_t*)(raw + (size_t)(r3 & 0xffu) * RB + offc);
+            const uint32_t rot = (ln >> 2) & 3u;
             if (!(r3 >> 16)) {
-                for (uint32_t qd = ln; qd < quads; qd += 64) {
-                    const uint32_t* rw = rc + qd;
-                    const uint32_t a0 = rw[0], a1 = rw[1], b0 = rw[64], b1 = rw[65];
-                    uint32_t* dst = row + 4 * qd;
-                    uint32_t w[7];
+                for (uint32_t g = ln; g < quads; g += 64) {
+                    uint32_t d[8];
 #pragma unroll
-                    for (int k = 0; k < 4; k++) w[k] = pair_word(b0, a0, k);
+                    for (int k = 0; k < 8; k++) d[k] = rc[g + 16 * k];
 #pragma unroll
-                    for (int k = 0; k < 3; k++) w[4 + k] = pair_word(b1, a1, k);
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        *(uint4*)(dst + (size_t)k * W) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+                    for (uint32_t bb = 0; bb < 4; bb++) {
+                        const uint32_t b = (bb + rot) & 3u;
+                        *(uint4*)(row + 4 * (4 * g + b)) = make_uint4(
+                            pair_word(d[1], d[0], b), pair_word(d[3], d[2], b), pair_word(d[5], d[4], b),
+                            pair_word(d[7], d[6], b));
+                    }
                 }
             } else {
                 const uint32_t* r1 = (const uint32_t*)(raw + (size_t)((r3 >> 8) & 0xffu) * RB + (offc1 & ~3u));
                 const uint32_t sh = offc1 & 3u;
-                for (uint32_t qd = ln; qd < quads; qd += 64) {
-                    const uint32_t* rw = rc + qd;
-                    const uint32_t a0 = rw[0], a1 = rw[1], b0 = rw[64], b1 = rw[65];
-                    const uint32_t* rv = r1 + qd;
-                    const uint32_t p0 = rv[0], p1 = rv[1], p2 = rv[2], q0 = rv[64], q1 = rv[65], q2 = rv[66];
-                    const uint32_t c0 = __builtin_amdgcn_alignbyte(p1, p0, sh);
-                    const uint32_t c1 = __builtin_amdgcn_alignbyte(p2, p1, sh);
-                    const uint32_t e0 = __builtin_amdgcn_alignbyte(q1, q0, sh);
-                    const uint32_t e1 = __builtin_amdgcn_alignbyte(q2, q1, sh);
-                    uint32_t* dst = row + 4 * qd;
-                    uint32_t w[7];
+                for (uint32_t g = ln; g < quads; g += 64) {
+                    uint32_t d[8], e[8];
 #pragma unroll
-                    for (int k = 0; k < 4; k++) w[k] = pair_word(b0, a0, k) + pair_word(e0, c0, k);
+                    for (int k = 0; k < 8; k++) {
+                        d[k] = rc[g + 16 * k];
+                        e[k] = __builtin_amdgcn_alignbyte(r1[g + 16 * k + 1], r1[g + 16 * k], sh);
+                    }
 #pragma unroll
-                    for (int k = 0; k < 3; k++) w[4 + k] = pair_word(b1, a1, k) + pair_word(e1, c1, k);
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        *(uint4*)(dst + (size_t)k * W) = make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+                    for (uint32_t bb = 0; bb < 4; bb++) {
+                        const uint32_t b = (bb + rot) & 3u;
+                        *(uint4*)(row + 4 * (4 * g + b)) = make_uint4(
+                            pair_word(d[1], d[0], b) + pair_word(e[1], e[0], b),
+                            pair_word(d[3], d[2], b) + pair_word(e[3], e[2], b),
+                            pair_word(d[5], d[4], b) + pair_word(e[5], e[4], b),
+                            pair_word(d[7], d[6], b) + pair_word(e[7], e[6], b));
+                    }
                 }
             }
         }
         __syncthreads();
         if (st + 1 < nstages) load_windows(st + 1);
         if (st + 2 < nstages) load_record(st + 2);
-        // slots in pairs: one v_add3_u32 adds both slots' u16 pairs (no half can
-        // carry within a phase: every slot adds at most 2 * maxval per half).
+        // slots in pairs: one v_add3_u32 adds both slots' u16 lanes (no lane can
+        // carry within a phase: every slot adds at most 2 * maxval per lane).
         // The wave's 8 read addresses per slot come from 16-byte broadcast reads
         // (16 bits each), fetched one pair ahead.
         const uint16_t* lw = (const uint16_t*)(sr + DDP_LAD_REC) + wave * DD_M;
@@ -600,58 +611,43 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
         }
     }
     widen();
-    // totals: low 16 bits from lo, the carries above them
+    // totals: low 16 bits from lo, the carries above them; acc32[m][k] is output
+    // t0 + lane + 64 k (dword j holds k = 2j low, 2j + 1 high)
     uint32_t acc32[DD_M][8];
 #pragma unroll
     for (int m = 0; m < DD_M; m++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            acc32[m][j] = (lo[m][j] & 0xffffu) + (((cy[m] >> (4 * j)) & 15u) << 16);
-            acc32[m][4 + j] = (lo[m][j] >> 16) + (((cy[m] >> (4 * j + 16)) & 15u) << 16);
+            acc32[m][2 * j] = (lo[m][j] & 0xffffu) + (((cy[m] >> (4 * j)) & 15u) << 16);
+            acc32[m][2 * j + 1] = (lo[m][j] >> 16) + (((cy[m] >> (4 * j + 16)) & 15u) << 16);
         }
     const int m0i = (int)m0;
+    uint32_t ln = threadIdx.x;  // as in load_record
+    asm volatile("" : "+v"(ln));
+    ln &= 63;
 #pragma unroll
     for (int m = 0; m < DD_M; m++) {
         const uint32_t d = d0 + wave * DD_M + m;
         double lsum = 0.0;
         if (d < ntrials) {
 #pragma unroll
-            for (int hh = 0; hh < 2; hh++) {
-                const uint32_t t = t0 + 256 * hh + 4 * lane;  // outputs t .. t + 3
-                const uint32_t* v = &acc32[m][4 * hh];
-                const size_t o = (size_t)d * out_pitch + t;
-                if (t + 3 < n) {
-                    lsum += (double)v[0] + (double)v[1] + (double)v[2] + (double)v[3];
-                    if (fp16) {
-                        __half2 h01 = __floats2half2_rn((float)((int)v[0] - m0i) * inv_s0,
-                                                        (float)((int)v[1] - m0i) * inv_s0);
-                        __half2 h23 = __floats2half2_rn((float)((int)v[2] - m0i) * inv_s0,
-                                                        (float)((int)v[3] - m0i) * inv_s0);
-                        uint2 pk;
-                        pk.x = __builtin_bit_cast(uint32_t, h01);
-                        pk.y = __builtin_bit_cast(uint32_t, h23);
-                        *(uint2*)((__half*)out + o) = pk;
-                    } else {
-                        *(float4*)((float*)out + o) =
-                            make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        if (t + j < n) {
-                            lsum += (double)v[j];
-                            if (fp16)
-                                ((__half*)out)[o + j] = __float2half_rn((float)((int)v[j] - m0i) * inv_s0);
-                            else
-                                ((float*)out)[o + j] = (float)v[j];
-                        }
+            for (int k = 0; k < 8; k++) {
+                const uint32_t t = t0 + ln + 64 * k;
+                if (t < n) {
+                    const uint32_t v = acc32[m][k];
+                    const size_t o = (size_t)d * out_pitch + t;
+                    lsum += (double)v;
+                    if (fp16)
+                        ((__half*)out)[o] = __float2half_rn((float)((int)v - m0i) * inv_s0);
+                    else
+                        ((float*)out)[o] = (float)v;
                 }
             }
         }
         // wave reduction of the tile's sum (integers < 2^53: exact in any order)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
-        if (lane == 0 && d < ntrials) partial[(size_t)d * ntiles + tile] = lsum;
+        if (ln == 0 && d < ntrials) partial[(size_t)d * ntiles + tile] = lsum;
     }
 }
 
@@ -661,21 +657,21 @@ size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
 }
 
 bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
-                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
-                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
-                       double* partial, const uint32_t* spread) {
+                      uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
+                      uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                      double* partial, const uint32_t* spread) {
     // 4-bit carry counters per output: every total must stay below 2^20;
     // 16-byte raw loads; 32-bit buffer offsets; 16-bit stage numbers
     if (nchans == 0 || ntrials == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
     if (pitch % 16 || pitch < 16 || pitch >= (1ull << 31) || nchans > 0xffff) return false;
-    // A row has W <= spread + 262 words (a trial reads words (e & ~3) - L ..
-    // + 255 of each copy); its raw windows are read up to byte spread + 556.
+    // A row has W <= spread + 3 + 64 + 3 words of 16 bytes; its raw windows are
+    // read up to byte spread + 560.
     uint32_t smax = 0;
     for (uint32_t c = 0; c < nchans; c++) smax = std::max(smax, spread[c]);
-    const uint64_t wmax = (uint64_t)smax + 3 + 3 + DD_TILE / 2;
+    const uint64_t wmax = (uint64_t)smax + 6 + DD_TILE / 8 + 3;
     const uint32_t RB = (uint32_t)(((uint64_t)smax + 600 + 1023) / 1024 * 1024);
     if (RB > DD_RAW_BYTES) return false;
-    const uint32_t windows = std::min<uint32_t>(DD_WAVES, DD_RAW_BYTES / RB);
+    const uint32_t windows = std::min<uint32_t>(DDP_MAX_WINDOWS, DD_RAW_BYTES / RB);
     if (windows < 2 || 16 * wmax > (uint64_t)DD_ROW_BYTES) return false;
     // a slot of kmax pair rows must fit the row area alone
     uint32_t kmax = DDP_KMAX;
@@ -695,7 +691,7 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan_shm);
     hipLaunchKernelGGL(k_dd_plan, dim3((unsigned)nblk), dim3(256), plan_shm, s, delays, nchans, ntrials,
                        kmax, windows, (uint32_t)DD_ROW_BYTES, bases, plan, pw);
-    // slots per phase: no 16-bit half can overflow (a slot adds at most 2 * maxval)
+    // slots per phase: no 16-bit lane can overflow (a slot adds at most 2 * maxval)
     uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
     phase = phase > 4096 ? 4096 : phase;
     if (phase < 2) return false;
