@@ -255,19 +255,36 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
     }
     __syncthreads();
     // channel pairs: distinct deltas and, per delta, the e range of channel c
+    // (the delays are read 16 trials at a time, all loads issued together)
+    constexpr int TB = 16;
     for (uint32_t u = threadIdx.x; u < npair; u += blockDim.x) {
         const uint32_t c = 2 * u;
         uint32_t K = 0;
-        int32_t del[DDP_KMAX], lo[DDP_KMAX], hi[DDP_KMAX];
-        if (c + 1 < nchans) {
-            for (uint32_t m = 0; m < (uint32_t)DD_TRIALS; m++) {
-                const int32_t dc = delay(m, c), dl = delay(m, c + 1) - dc, e = dc - sbase[c];
+        bool split = c + 1 >= nchans;
+        int32_t del[DDP_KMAX] = {0, 0, 0}, lo[DDP_KMAX] = {0, 0, 0}, hi[DDP_KMAX] = {0, 0, 0};
+        int32_t emin[2] = {1 << 30, 1 << 30}, emax[2] = {0, 0};
+        const int32_t b0 = sbase[c], b1 = c + 1 < nchans ? sbase[c + 1] : 0;
+        for (uint32_t m0 = 0; m0 < (uint32_t)DD_TRIALS; m0 += TB) {
+            int32_t dc[TB], d1[TB];
+#pragma unroll
+            for (int i = 0; i < TB; i++) {
+                dc[i] = delay(m0 + i, c);
+                d1[i] = c + 1 < nchans ? delay(m0 + i, c + 1) : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < TB; i++) {
+                const int32_t e = dc[i] - b0, dl = d1[i] - dc[i];
+                emin[0] = min(emin[0], e);
+                emax[0] = max(emax[0], e);
+                emin[1] = min(emin[1], d1[i] - b1);
+                emax[1] = max(emax[1], d1[i] - b1);
+                if (split) continue;
                 uint32_t k = 0;
                 while (k < K && del[k] != dl) k++;
                 if (k == K) {
                     if (K == kmax) {
-                        K = 0;
-                        break;
+                        split = true;
+                        continue;
                     }
                     del[K] = dl;
                     lo[K] = hi[K] = e;
@@ -278,24 +295,17 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
                 }
             }
         }
+        if (split) K = 0;
         ukind[u] = K;
-        for (uint32_t k = 0; k < K; k++) {
+#pragma unroll
+        for (int k = 0; k < DDP_KMAX; k++) {
             udel[u * DDP_KMAX + k] = del[k];
-            uLW[u * DDP_KMAX + k] = row_lw((uint32_t)lo[k], (uint32_t)hi[k]);
+            uLW[u * DDP_KMAX + k] = (uint32_t)k < K ? row_lw((uint32_t)lo[k], (uint32_t)hi[k])
+                                    : (K == 0 && k < 2) ? row_lw((uint32_t)emin[k], (uint32_t)emax[k]) : 0u;
         }
-        if (K == 0)  // single-channel rows of c (and c + 1)
-            for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
-                int32_t emin = 1 << 30, emax = 0;
-                for (uint32_t m = 0; m < (uint32_t)DD_TRIALS; m++) {
-                    const int32_t e = delay(m, c + j) - sbase[c + j];
-                    emin = min(emin, e);
-                    emax = max(emax, e);
-                }
-                uLW[u * DDP_KMAX + j] = row_lw((uint32_t)emin, (uint32_t)emax);
-            }
     }
     __syncthreads();
-    // stages, in channel order
+    // stages, in channel order (one thread; each pair's data is read up front)
     if (threadIdx.x == 0) {
         uint32_t nst = 0, nslot_all = 0;
         uint32_t nwin = 0, nslot = 0, nrow = 0, bytes = 0;
@@ -329,28 +339,38 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
         };
         for (uint32_t u = 0; u < npair; u++) {
             const uint32_t c = 2 * u, K = ukind[u];
+            uint32_t LW[DDP_KMAX];
+            int32_t dl[DDP_KMAX];
+#pragma unroll
+            for (int k = 0; k < DDP_KMAX; k++) {
+                LW[k] = uLW[u * DDP_KMAX + k];
+                dl[k] = udel[u * DDP_KMAX + k];
+            }
+            const int32_t sb0 = sbase[c], sb1 = c + 1 < nchans ? sbase[c + 1] : 0;
             if (K) {
                 uint32_t need = 0;
-                for (uint32_t k = 0; k < K; k++) need += 16 * (uLW[u * DDP_KMAX + k] >> 16);
+#pragma unroll
+                for (int k = 0; k < DDP_KMAX; k++) need += (uint32_t)k < K ? 16 * (LW[k] >> 16) : 0u;
                 if (!fits(2, K, need)) close();
                 uslot[u] = nslot_all;
                 st[4 + nwin] = c;
                 st[5 + nwin] = c + 1;
-                const int32_t shift = sbase[c + 1] - sbase[c];
-                for (uint32_t k = 0; k < K; k++) {
-                    const uint32_t L = uLW[u * DDP_KMAX + k] & 0xffffu, W = uLW[u * DDP_KMAX + k] >> 16;
-                    const int32_t o1 = (int32_t)ddp_mis(sbase[c + 1]) + (int32_t)L + udel[u * DDP_KMAX + k] - shift;
-                    put_row(u * DDP_KMAX + k, W, ddp_mis(sbase[c]) + L, (uint32_t)o1, nwin, nwin + 1, 1);
+#pragma unroll
+                for (int k = 0; k < DDP_KMAX; k++) {
+                    if ((uint32_t)k >= K) break;
+                    const uint32_t L = LW[k] & 0xffffu, W = LW[k] >> 16;
+                    const int32_t o1 = (int32_t)ddp_mis(sb1) + (int32_t)L + dl[k] - (sb1 - sb0);
+                    put_row(u * DDP_KMAX + k, W, ddp_mis(sb0) + L, (uint32_t)o1, nwin, nwin + 1, 1);
                 }
                 nwin += 2;
                 put_slot();
             } else {
                 for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
-                    const uint32_t L = uLW[u * DDP_KMAX + j] & 0xffffu, W = uLW[u * DDP_KMAX + j] >> 16;
+                    const uint32_t L = LW[j] & 0xffffu, W = LW[j] >> 16;
                     if (!fits(1, 1, 16 * W)) close();
                     if (j == 0) uslot[u] = nslot_all;
                     st[4 + nwin] = c + j;
-                    put_row(u * DDP_KMAX + j, W, ddp_mis(sbase[c + j]) + L, 0, nwin, 0, 0);
+                    put_row(u * DDP_KMAX + j, W, ddp_mis(j ? sb1 : sb0) + L, 0, nwin, 0, 0);
                     nwin++;
                     put_slot();
                 }
@@ -360,30 +380,46 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
         pb[0] = nst;
     }
     __syncthreads();
-    // every trial's LDS read address per slot
+    // every trial's LDS read address per slot: one pair per thread, 16 trials
+    // at a time
     auto lad = [&](uint32_t slot) {
         const uint32_t loc = sloc[slot];
         return (uint16_t*)(pb + DDP_PLAN_HEAD + (size_t)(loc & 0xffffu) * DDP_STAGE_PITCH + DDP_LAD_REC) +
                (size_t)(loc >> 16) * DD_TRIALS;
     };
-    for (uint32_t idx = threadIdx.x; idx < npair * DD_TRIALS; idx += blockDim.x) {
-        const uint32_t u = idx / DD_TRIALS, m = idx % DD_TRIALS, c = 2 * u, K = ukind[u];
-        if (K) {
-            const int32_t dc = delay(m, c), dl = delay(m, c + 1) - dc;
-            const uint32_t e = (uint32_t)(dc - sbase[c]);
-            uint32_t k = 0;
-            while (k + 1 < K && udel[u * DDP_KMAX + k] != dl) k++;
-            const uint32_t L = uLW[u * DDP_KMAX + k] & 0xffffu;
-            lad(uslot[u])[m] = (uint16_t)(uRO[u * DDP_KMAX + k] + (e - L) * 16);
-        } else {
-            for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
-                const uint32_t e = (uint32_t)(delay(m, c + j) - sbase[c + j]);
-                const uint32_t L = uLW[u * DDP_KMAX + j] & 0xffffu;
-                lad(uslot[u] + j)[m] = (uint16_t)(uRO[u * DDP_KMAX + j] + (e - L) * 16);
+    for (uint32_t u = threadIdx.x; u < npair; u += blockDim.x) {
+        const uint32_t c = 2 * u, K = ukind[u];
+        const int32_t b0 = sbase[c], b1 = c + 1 < nchans ? sbase[c + 1] : 0;
+        uint16_t* l0 = lad(uslot[u]);
+        uint16_t* l1 = K == 0 && c + 1 < nchans ? lad(uslot[u] + 1) : l0;
+        for (uint32_t m0 = 0; m0 < (uint32_t)DD_TRIALS; m0 += TB) {
+            int32_t dc[TB], d1[TB];
+#pragma unroll
+            for (int i = 0; i < TB; i++) {
+                dc[i] = delay(m0 + i, c);
+                d1[i] = c + 1 < nchans ? delay(m0 + i, c + 1) : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < TB; i++) {
+                const uint32_t m = m0 + i, e = (uint32_t)(dc[i] - b0);
+                if (K) {
+                    const int32_t dl = d1[i] - dc[i];
+                    uint32_t k = 0;
+                    while (k + 1 < K && udel[u * DDP_KMAX + k] != dl) k++;
+                    const uint32_t L = uLW[u * DDP_KMAX + k] & 0xffffu;
+                    l0[m] = (uint16_t)(uRO[u * DDP_KMAX + k] + (e - L) * 16);
+                } else {
+                    l0[m] = (uint16_t)(uRO[u * DDP_KMAX] + (e - (uLW[u * DDP_KMAX] & 0xffffu)) * 16);
+                    if (c + 1 < nchans) {
+                        const uint32_t e1 = (uint32_t)(d1[i] - b1);
+                        l1[m] = (uint16_t)(uRO[u * DDP_KMAX + 1] + (e1 - (uLW[u * DDP_KMAX + 1] & 0xffffu)) * 16);
+                    }
+                }
             }
         }
     }
 }
+
 
 __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_slots,

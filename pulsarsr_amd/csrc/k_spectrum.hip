@@ -576,16 +576,15 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
         const uint32_t ku = (uint32_t)k;
         const float r0 = raw_at(raw, fp16, row + ku - 1), r1 = raw_at(raw, fp16, row + ku),
                     r2 = raw_at(raw, fp16, row + ku + 1);
-        const float s0 = scl[(ku - 1) >> 6], s1 = scl[ku >> 6], s2 = scl[(ku + 1) >> 6];
+        const float s1 = scl[ku >> 6], s2 = scl[(ku + 1) >> 6];
         offset = interp_offset((double)r0, (double)r1, (double)r2);
         const float pwk = whiten_val(r1, s1, ku, a);
-        // j = 1 sums the bin at round(k + offset), one of k - 1, k, k + 1
-        uint64_t i1 = (uint64_t)round((double)ku + offset);
-        if (i1 > nbins - 1) i1 = nbins - 1;
-        const float w1 = i1 < ku ? whiten_val(r0, s0, ku - 1, a)
-                                 : (i1 == ku ? pwk : whiten_val(r2, s2, ku + 1, a));
+        // j = 1 sums the bin at round(k + offset): offset is in [-0.5, 0.5] and
+        // round() takes halves away from zero, so that is k or k + 1 (<= nbins - 1)
+        const bool up = round((double)ku + offset) > (double)ku;
+        const float w1 = up ? whiten_val(r2, s2, ku + 1, a) : pwk;
         S1 = (double)w1;
-        sc[0][tid] = (S1 - 1.0) / a.sqrt_h[0];
+        sc[0][tid] = S1 - 1.0;  // / sqrt(1): exact
         listed = a.p.nlevels > 1 && k <= (int64_t)a.p.end_h[1] && pwk >= 4.0f;
     }
     if (listed) {
@@ -616,7 +615,8 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
                 const uint32_t h = 1u << l;
                 if (kk > (int64_t)a.p.end_h[l]) break;
                 for (; j <= h; j++) S += (double)hv[tid][j - 2];
-                sc[l][t] = (S - (double)h) / a.sqrt_h[l];
+                // sqrt(4) and sqrt(16) are powers of two: the product is the exact quotient
+                sc[l][t] = (l & 1) ? (S - (double)h) / a.sqrt_h[l] : ldexp(S - (double)h, -(int)(l >> 1));
             }
         }
         __syncthreads();
