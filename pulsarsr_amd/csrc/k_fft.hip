@@ -291,12 +291,14 @@ __device__ __forceinline__ float2 load_pair(const void* series, size_t row, uint
 
 // Stored spectrum value: the power (FP32) or the amplitude sqrt(power / n)
 // (FP16, which keeps every value in half range); returns what the whitening
-// floor sums.
+// floor sums.  The FP16 amplitude takes the hardware square root (within 1 ulp
+// of f32, far below the half-precision rounding that follows) instead of the
+// correctly rounded sequence, a seventh of the instructions.
 template <bool F16>
 __device__ __forceinline__ double store_power(void* spec, size_t idx, float p, float inv_n) {
     if (F16) {
         const float q = p * inv_n;
-        ((__half*)spec)[idx] = __float2half_rn(sqrtf(q));
+        ((__half*)spec)[idx] = __float2half_rn(__builtin_amdgcn_sqrtf(q));
         return (double)q;
     }
     ((float*)spec)[idx] = p;
