@@ -499,12 +499,21 @@ void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp1
 }
 
 // ----------------------------------------------------- harmonic summing
-__device__ __forceinline__ float raw_at(const void* raw, int fp16, size_t idx) {
-    if (fp16) {
-        const float a = __half2float(((const __half*)raw)[idx]);
+// raw spectrum word (FP16 amplitude bits or FP32 power bits) and its power;
+// loads and conversions are kept apart so that several loads can be in flight
+template <bool F16>
+__device__ __forceinline__ uint32_t raw_word(const void* raw, size_t idx) {
+    if constexpr (F16) return ((const uint16_t*)raw)[idx];
+    else return ((const uint32_t*)raw)[idx];
+}
+template <bool F16>
+__device__ __forceinline__ float raw_power(uint32_t w) {
+    if constexpr (F16) {
+        const float a = __half2float(__ushort_as_half((unsigned short)w));
         return a * a;
+    } else {
+        return __uint_as_float(w);
     }
-    return ((const float*)raw)[idx];
 }
 
 struct HarmArgs {
@@ -537,8 +546,11 @@ constexpr int HS_T = 256;       // threads: scores for HS_T bins
 constexpr int HS_OUT = HS_T - 2;  // peak tests per workgroup
 constexpr int HS_NJ = 15;         // harmonics j = 2 .. 16
 constexpr int HS_LIST = 64;       // listed bins summed per round
+constexpr int HS_TILES = 4;       // tiles of HS_OUT bins per workgroup, the next one's inputs prefetched
 
-// One workgroup scores HS_T consecutive bins.  Level 0 (j = 1) is computed for
+// One workgroup scores HS_TILES tiles of HS_T consecutive bins in turn (tiles
+// overlap by two bins, the peak test's neighbours), loading the next tile's
+// level-0 inputs while the current one is scored.  Level 0 (j = 1) is computed for
 // every bin; higher levels only for bins whose whitened power reaches 4
 // (lib.rs:97-99), typically a few percent.  Those bins are compacted into a
 // list; HS_LIST of them at a time, their (bin, harmonic) loads are spread over
@@ -546,12 +558,13 @@ constexpr int HS_LIST = 64;       // listed bins summed per round
 // per listed bin then adds its harmonics in order j = 2, 3, ... in f64, as the
 // reference does.  The small list keeps the LDS footprint low enough for eight
 // workgroups per CU.
-__global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, int fp16,
+template <bool F16>
+__global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
                                                  uint64_t spec_pitch, uint32_t nbins,
                                                  uint32_t nblocks,
                                                  const float* __restrict__ scales, HarmArgs a,
                                                  RawPeak* __restrict__ peaks,
-                                                 uint32_t* __restrict__ peak_counts) {
+                                                 uint32_t* __restrict__ peak_counts, uint32_t ntiles) {
     __shared__ double sc[5][HS_T];
     __shared__ double lpos[HS_T];   // k + offset of listed bins
     __shared__ double lsum[HS_T];   // their j = 1 term
@@ -563,80 +576,108 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw, i
     const size_t row = (size_t)trial * spec_pitch;
     const float* scl = scales + (size_t)trial * nblocks;
     const int tid = threadIdx.x;
-    const int64_t k = (int64_t)a.p.first - 1 + (int64_t)wb.x * HS_OUT + tid;
     const double NEG_INF = -__builtin_inf();
-    double offset = 0.0;
+    // this thread's bin of tile `tile` and its level-0 inputs (raw k-1, k, k+1
+    // and the scales of k, k+1), loaded one tile ahead
+    struct In {
+        uint32_t r0, r1, r2;
+        float s1, s2;
+    };
+    auto bin_of = [&](uint32_t tile) { return (int64_t)a.p.first - 1 + (int64_t)tile * HS_OUT + tid; };
+    auto load_in = [&](uint32_t tile) {
+        In v{0u, 0u, 0u, 1.f, 1.f};
+        const int64_t k = bin_of(tile);
+        if (tile < ntiles && k >= (int64_t)a.p.first && k <= (int64_t)a.p.end_h[0]) {
+            const uint32_t ku = (uint32_t)k;
+            v.r0 = raw_word<F16>(raw, row + ku - 1);
+            v.r1 = raw_word<F16>(raw, row + ku);
+            v.r2 = raw_word<F16>(raw, row + ku + 1);
+            v.s1 = scl[ku >> 6];
+            v.s2 = scl[(ku + 1) >> 6];
+        }
+        return v;
+    };
+    In cur = load_in(wb.x * HS_TILES);
+    for (uint32_t it = 0; it < (uint32_t)HS_TILES; it++) {
+        const uint32_t tile = wb.x * HS_TILES + it;
+        if (tile >= ntiles) break;
+        const In nxt = load_in(tile + 1);  // in flight while this tile is scored
+        const int64_t k = bin_of(tile);
+        double offset = 0.0;
 #pragma unroll
-    for (int l = 0; l < 5; l++) sc[l][tid] = NEG_INF;
-    if (tid == 0) nlist = 0;
-    __syncthreads();
-    bool listed = false;
-    double S1 = 0.0;
-    if (k >= (int64_t)a.p.first && k <= (int64_t)a.p.end_h[0]) {
-        const uint32_t ku = (uint32_t)k;
-        const float r0 = raw_at(raw, fp16, row + ku - 1), r1 = raw_at(raw, fp16, row + ku),
-                    r2 = raw_at(raw, fp16, row + ku + 1);
-        const float s1 = scl[ku >> 6], s2 = scl[(ku + 1) >> 6];
-        offset = interp_offset((double)r0, (double)r1, (double)r2);
-        const float pwk = whiten_val(r1, s1, ku, a);
-        // j = 1 sums the bin at round(k + offset): offset is in [-0.5, 0.5] and
-        // round() takes halves away from zero, so that is k or k + 1 (<= nbins - 1)
-        const bool up = round((double)ku + offset) > (double)ku;
-        const float w1 = up ? whiten_val(r2, s2, ku + 1, a) : pwk;
-        S1 = (double)w1;
-        sc[0][tid] = S1 - 1.0;  // / sqrt(1): exact
-        listed = a.p.nlevels > 1 && k <= (int64_t)a.p.end_h[1] && pwk >= 4.0f;
-    }
-    if (listed) {
-        const uint32_t slot = atomicAdd(&nlist, 1u);
-        lpos[slot] = (double)k + offset;
-        lsum[slot] = S1;
-        lbin[slot] = (uint8_t)tid;
-    }
-    __syncthreads();
-    const uint32_t G = nlist;
-    for (uint32_t g0 = 0; g0 < G; g0 += HS_LIST) {
-        const uint32_t ng = min((uint32_t)HS_LIST, G - g0);
-        for (uint32_t it = tid; it < ng * HS_NJ; it += HS_T) {
-            const uint32_t g = it / HS_NJ, jj = it - g * HS_NJ;
-            const double position = lpos[g0 + g] * (double)(jj + 2);
-            uint64_t index = (uint64_t)round(position);
-            if (index > nbins - 1) index = nbins - 1;
-            const uint32_t iu = (uint32_t)index;
-            hv[g][jj] = whiten_val(raw_at(raw, fp16, row + iu), scl[iu >> 6], iu, a);
+        for (int l = 0; l < 5; l++) sc[l][tid] = NEG_INF;
+        if (tid == 0) nlist = 0;
+        __syncthreads();
+        bool listed = false;
+        double S1 = 0.0;
+        if (k >= (int64_t)a.p.first && k <= (int64_t)a.p.end_h[0]) {
+            const uint32_t ku = (uint32_t)k;
+            const float p0 = raw_power<F16>(cur.r0), p1 = raw_power<F16>(cur.r1),
+                        p2 = raw_power<F16>(cur.r2);
+            offset = interp_offset((double)p0, (double)p1, (double)p2);
+            const float pwk = whiten_val(p1, cur.s1, ku, a);
+            // j = 1 sums the bin at round(k + offset): offset is in [-0.5, 0.5] and
+            // round() takes halves away from zero, so that is k or k + 1 (<= nbins - 1)
+            const bool up = round((double)ku + offset) > (double)ku;
+            const float w1 = up ? whiten_val(p2, cur.s2, ku + 1, a) : pwk;
+            S1 = (double)w1;
+            sc[0][tid] = S1 - 1.0;  // / sqrt(1): exact
+            listed = a.p.nlevels > 1 && k <= (int64_t)a.p.end_h[1] && pwk >= 4.0f;
+        }
+        if (listed) {
+            const uint32_t slot = atomicAdd(&nlist, 1u);
+            lpos[slot] = (double)k + offset;
+            lsum[slot] = S1;
+            lbin[slot] = (uint8_t)tid;
         }
         __syncthreads();
-        if ((uint32_t)tid < ng) {
-            const uint32_t t = lbin[g0 + tid];
-            const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)wb.x * HS_OUT + t;
-            double S = lsum[g0 + tid];
-            uint32_t j = 2;
-            for (uint32_t l = 1; l < a.p.nlevels; l++) {
-                const uint32_t h = 1u << l;
-                if (kk > (int64_t)a.p.end_h[l]) break;
-                for (; j <= h; j++) S += (double)hv[tid][j - 2];
-                // sqrt(4) and sqrt(16) are powers of two: the product is the exact quotient
-                sc[l][t] = (l & 1) ? (S - (double)h) / a.sqrt_h[l] : ldexp(S - (double)h, -(int)(l >> 1));
+        const uint32_t G = nlist;
+        for (uint32_t g0 = 0; g0 < G; g0 += HS_LIST) {
+            const uint32_t ng = min((uint32_t)HS_LIST, G - g0);
+            for (uint32_t q = tid; q < ng * HS_NJ; q += HS_T) {
+                const uint32_t g = q / HS_NJ, jj = q - g * HS_NJ;
+                const double position = lpos[g0 + g] * (double)(jj + 2);
+                uint64_t index = (uint64_t)round(position);
+                if (index > nbins - 1) index = nbins - 1;
+                const uint32_t iu = (uint32_t)index;
+                const uint32_t w = raw_word<F16>(raw, row + iu);
+                hv[g][jj] = whiten_val(raw_power<F16>(w), scl[iu >> 6], iu, a);
+            }
+            __syncthreads();
+            if ((uint32_t)tid < ng) {
+                const uint32_t t = lbin[g0 + tid];
+                const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)tile * HS_OUT + t;
+                double S = lsum[g0 + tid];
+                uint32_t j = 2;
+                for (uint32_t l = 1; l < a.p.nlevels; l++) {
+                    const uint32_t h = 1u << l;
+                    if (kk > (int64_t)a.p.end_h[l]) break;
+                    for (; j <= h; j++) S += (double)hv[tid][j - 2];
+                    // sqrt(4) and sqrt(16) are powers of two: the product is the exact quotient
+                    sc[l][t] = (l & 1) ? (S - (double)h) / a.sqrt_h[l] : ldexp(S - (double)h, -(int)(l >> 1));
+                }
+            }
+            __syncthreads();
+        }
+        if (tid >= 1 && tid <= HS_OUT && k >= (int64_t)a.p.first + 1) {
+            for (uint32_t l = 0; l < a.p.nlevels; l++) {
+                if (k > (int64_t)a.p.end_h[l] - 1) break;
+                const double s = sc[l][tid];
+                if (s >= a.p.threshold && s > sc[l][tid - 1] && s >= sc[l][tid + 1]) {
+                    const uint32_t slot = atomicAdd(&peak_counts[trial], 1u);
+                    if (slot < a.p.peak_cap) {
+                        RawPeak rp;
+                        rp.score = s;
+                        rp.offset = offset;
+                        rp.k = (uint32_t)k;
+                        rp.h = 1u << l;
+                        peaks[(size_t)trial * a.p.peak_cap + slot] = rp;
+                    }
+                }
             }
         }
-        __syncthreads();
-    }
-    if (tid < 1 || tid > HS_OUT) return;
-    if (k < (int64_t)a.p.first + 1) return;
-    for (uint32_t l = 0; l < a.p.nlevels; l++) {
-        if (k > (int64_t)a.p.end_h[l] - 1) break;
-        const double s = sc[l][tid];
-        if (s >= a.p.threshold && s > sc[l][tid - 1] && s >= sc[l][tid + 1]) {
-            const uint32_t slot = atomicAdd(&peak_counts[trial], 1u);
-            if (slot < a.p.peak_cap) {
-                RawPeak rp;
-                rp.score = s;
-                rp.offset = offset;
-                rp.k = (uint32_t)k;
-                rp.h = 1u << l;
-                peaks[(size_t)trial * a.p.peak_cap + slot] = rp;
-            }
-        }
+        __syncthreads();  // the next tile rewrites the scores
+        cur = nxt;
     }
 }
 
@@ -652,9 +693,14 @@ void launch_harmsum(hipStream_t s, const SpecGeom& g, const SearchParams& p, con
     static const double SQ[5] = {1.0, 1.4142135623730951, 2.0, 2.8284271247461903, 4.0};
     for (int l = 0; l < 5; l++) a.sqrt_h[l] = SQ[l];
     const uint32_t span = p.end_h[0] - p.first + 1;
-    dim3 grid((span + HS_OUT - 1) / HS_OUT, ntrials);
-    hipLaunchKernelGGL(k_harmsum, grid, dim3(HS_T), 0, s, raw_nat, (int)p.fp16, g.spec_pitch,
-                       g.nbins, g.nblocks, scales, a, peaks, peak_counts);
+    const uint32_t ntiles = (span + HS_OUT - 1) / HS_OUT;
+    dim3 grid((ntiles + HS_TILES - 1) / HS_TILES, ntrials);
+    if (p.fp16)
+        hipLaunchKernelGGL(k_harmsum<true>, grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch,
+                           g.nbins, g.nblocks, scales, a, peaks, peak_counts, ntiles);
+    else
+        hipLaunchKernelGGL(k_harmsum<false>, grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch,
+                           g.nbins, g.nblocks, scales, a, peaks, peak_counts, ntiles);
 }
 
 // ------------------------------------------------------------- ranking
