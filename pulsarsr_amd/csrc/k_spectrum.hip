@@ -577,24 +577,34 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
     const float* scl = scales + (size_t)trial * nblocks;
     const int tid = threadIdx.x;
     const double NEG_INF = -__builtin_inf();
-    // this thread's bin of tile `tile` and its level-0 inputs (raw k-1, k, k+1
-    // and the scales of k, k+1), loaded one tile ahead
+    // this thread's bin of tile `tile` and its level-0 inputs, loaded one tile
+    // ahead: raw k-1 .. k+1 (one 8-byte load of four FP16 words, or three FP32
+    // words; k clamped into range so that the loads need no branch and stay in
+    // flight) and the scales of k and k + 1
     struct In {
         uint32_t r0, r1, r2;
         float s1, s2;
     };
     auto bin_of = [&](uint32_t tile) { return (int64_t)a.p.first - 1 + (int64_t)tile * HS_OUT + tid; };
     auto load_in = [&](uint32_t tile) {
-        In v{0u, 0u, 0u, 1.f, 1.f};
+        In v;
         const int64_t k = bin_of(tile);
-        if (tile < ntiles && k >= (int64_t)a.p.first && k <= (int64_t)a.p.end_h[0]) {
-            const uint32_t ku = (uint32_t)k;
-            v.r0 = raw_word<F16>(raw, row + ku - 1);
-            v.r1 = raw_word<F16>(raw, row + ku);
-            v.r2 = raw_word<F16>(raw, row + ku + 1);
-            v.s1 = scl[ku >> 6];
-            v.s2 = scl[(ku + 1) >> 6];
+        const uint32_t ku = (uint32_t)min(max(k, (int64_t)a.p.first), (int64_t)a.p.end_h[0]);
+        if constexpr (F16) {
+            // words ku-1 .. ku+2; ku + 2 <= nbins < spec_pitch
+            uint64_t q;
+            __builtin_memcpy(&q, (const uint16_t*)raw + row + ku - 1, 8);
+            v.r0 = (uint32_t)q;
+            v.r1 = (uint32_t)(q >> 16);
+            v.r2 = (uint32_t)(q >> 32);
+        } else {
+            const uint32_t* r = (const uint32_t*)raw + row + ku;
+            v.r0 = r[-1];
+            v.r1 = r[0];
+            v.r2 = r[1];
         }
+        v.s1 = scl[ku >> 6];
+        v.s2 = scl[(ku + 1) >> 6];
         return v;
     };
     In cur = load_in(wb.x * HS_TILES);
@@ -612,8 +622,8 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
         double S1 = 0.0;
         if (k >= (int64_t)a.p.first && k <= (int64_t)a.p.end_h[0]) {
             const uint32_t ku = (uint32_t)k;
-            const float p0 = raw_power<F16>(cur.r0), p1 = raw_power<F16>(cur.r1),
-                        p2 = raw_power<F16>(cur.r2);
+            // (raw_power<true> reads the low 16 bits of the word)
+            const float p0 = raw_power<F16>(cur.r0), p1 = raw_power<F16>(cur.r1), p2 = raw_power<F16>(cur.r2);
             offset = interp_offset((double)p0, (double)p1, (double)p2);
             const float pwk = whiten_val(p1, cur.s1, ku, a);
             // j = 1 sums the bin at round(k + offset): offset is in [-0.5, 0.5] and
@@ -637,9 +647,9 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
             for (uint32_t q = tid; q < ng * HS_NJ; q += HS_T) {
                 const uint32_t g = q / HS_NJ, jj = q - g * HS_NJ;
                 const double position = lpos[g0 + g] * (double)(jj + 2);
-                uint64_t index = (uint64_t)round(position);
-                if (index > nbins - 1) index = nbins - 1;
-                const uint32_t iu = (uint32_t)index;
+                // round(position) >= 0 clamped to nbins - 1 (any u64 clamp of the
+                // rounded value gives the same bin)
+                const uint32_t iu = (uint32_t)fmin(round(position), (double)(nbins - 1));
                 const uint32_t w = raw_word<F16>(raw, row + iu);
                 hv[g][jj] = whiten_val(raw_power<F16>(w), scl[iu >> 6], iu, a);
             }
@@ -647,23 +657,33 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
             if ((uint32_t)tid < ng) {
                 const uint32_t t = lbin[g0 + tid];
                 const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)tile * HS_OUT + t;
+                float v[HS_NJ];
+#pragma unroll
+                for (int j = 0; j < HS_NJ; j++) v[j] = hv[tid][j];
+                // j = 2, 3, ... added in order; level l (h = 2^l) is scored after
+                // j = h while k is within its end bin (end bins fall with l)
                 double S = lsum[g0 + tid];
-                uint32_t j = 2;
-                for (uint32_t l = 1; l < a.p.nlevels; l++) {
-                    const uint32_t h = 1u << l;
-                    if (kk > (int64_t)a.p.end_h[l]) break;
-                    for (; j <= h; j++) S += (double)hv[tid][j - 2];
-                    // sqrt(4) and sqrt(16) are powers of two: the product is the exact quotient
-                    sc[l][t] = (l & 1) ? (S - (double)h) / a.sqrt_h[l] : ldexp(S - (double)h, -(int)(l >> 1));
+#pragma unroll
+                for (int l = 1; l < 5; l++) {
+                    const int h = 1 << l;
+#pragma unroll
+                    for (int j = (h >> 1) + 1; j <= h; j++) S += (double)v[j - 2];
+                    if ((uint32_t)l < a.p.nlevels && kk <= (int64_t)a.p.end_h[l])
+                        // sqrt(4) and sqrt(16) are powers of two: the product is the exact quotient
+                        sc[l][t] = (l & 1) ? (S - (double)h) / a.sqrt_h[l] : ldexp(S - (double)h, -(l >> 1));
                 }
             }
             __syncthreads();
         }
         if (tid >= 1 && tid <= HS_OUT && k >= (int64_t)a.p.first + 1) {
-            for (uint32_t l = 0; l < a.p.nlevels; l++) {
-                if (k > (int64_t)a.p.end_h[l] - 1) break;
-                const double s = sc[l][tid];
-                if (s >= a.p.threshold && s > sc[l][tid - 1] && s >= sc[l][tid + 1]) {
+            double sl[5];
+#pragma unroll
+            for (int l = 0; l < 5; l++) sl[l] = sc[l][tid];
+#pragma unroll
+            for (int l = 0; l < 5; l++) {
+                const double s = sl[l];
+                if ((uint32_t)l < a.p.nlevels && k <= (int64_t)a.p.end_h[l] - 1 && s >= a.p.threshold &&
+                    s > sc[l][tid - 1] && s >= sc[l][tid + 1]) {
                     const uint32_t slot = atomicAdd(&peak_counts[trial], 1u);
                     if (slot < a.p.peak_cap) {
                         RawPeak rp;
