@@ -247,7 +247,12 @@ __device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t n
             }
         T++;
     }
-    // exchange across the cut until max(left) <= min(right)
+    // exchange across the cut until max(left) <= min(right): the block holding
+    // the largest key left of the cut gives it up, the block holding the
+    // smallest key right of the cut takes it in.  Branch-free: both blocks are
+    // selected arithmetically, their two new cut neighbours are read together
+    // (addresses clamped into the block, out-of-range reads replaced by the
+    // sentinels), and the five cut states are updated with selects.
     for (int it = 0; it < 1024; it++) {
         int jl = 0, ju = 0;
         uint32_t lmax = L[0], umin = U[0];
@@ -263,18 +268,26 @@ __device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t n
             }
         }
         if (lmax <= umin) break;
+        int pl = p[0], pu = p[0], nu = nn[0];
+#pragma unroll
+        for (int j = 1; j < 5; j++) {
+            pl = j == jl ? p[j] : pl;
+            pu = j == ju ? p[j] : pu;
+            nu = j == ju ? nn[j] : nu;
+        }
+        // jl != ju (within one sorted block L <= U); new cut: p[jl] - 1, p[ju] + 1
+        const uint32_t kl = s[(lb + jl) * STRIDE + (pl >= 2 ? pl - 2 : 0)];
+        const uint32_t ku = s[(lb + ju) * STRIDE + (pu + 1 < 64 ? pu + 1 : 63)];
+        const uint32_t newL = pl >= 2 ? kl + 1u : 0u;
+        const uint32_t newU = pu + 1 < nu ? ku + 1u : KEY_INF;
 #pragma unroll
         for (int j = 0; j < 5; j++) {
-            if (j == jl) {
-                U[j] = L[j];
-                p[j]--;
-                L[j] = p[j] > 0 ? A[j][p[j] - 1] + 1u : 0u;
-            }
-            if (j == ju) {
-                L[j] = U[j];
-                p[j]++;
-                U[j] = p[j] < nn[j] ? A[j][p[j]] + 1u : KEY_INF;
-            }
+            const bool isl = j == jl, isu = j == ju;
+            p[j] += (isu ? 1 : 0) - (isl ? 1 : 0);
+            const uint32_t l2 = isl ? newL : (isu ? umin : L[j]);
+            const uint32_t u2 = isl ? lmax : (isu ? newU : U[j]);
+            L[j] = l2;
+            U[j] = u2;
         }
     }
     uint32_t umin = U[0];
