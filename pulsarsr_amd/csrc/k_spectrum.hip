@@ -248,10 +248,13 @@ __device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t n
         T++;
     }
     // exchange across the cut until max(left) <= min(right): the block holding
-    // the largest key left of the cut gives it up, the block holding the
-    // smallest key right of the cut takes it in.  Branch-free: both blocks are
-    // selected arithmetically, their two new cut neighbours are read together
-    // (addresses clamped into the block, out-of-range reads replaced by the
+    // the largest key left of the cut gives up its top keys, the block holding
+    // the smallest key right of the cut takes in its bottom keys -- 4 or 2 at a
+    // time while every key moved right is larger than every key moved left
+    // (each such step only swaps inverted pairs, so the search still ends at
+    // the unique median cut), else 1.  Branch-free: both blocks are selected
+    // arithmetically, the eight keys next to their cuts are read together
+    // (indices clamped into the block, out-of-range keys replaced by the
     // sentinels), and the five cut states are updated with selects.
     for (int it = 0; it < 1024; it++) {
         int jl = 0, ju = 0;
@@ -275,17 +278,32 @@ __device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t n
             pu = j == ju ? p[j] : pu;
             nu = j == ju ? nn[j] : nu;
         }
-        // jl != ju (within one sorted block L <= U); new cut: p[jl] - 1, p[ju] + 1
-        const uint32_t kl = s[(lb + jl) * STRIDE + (pl >= 2 ? pl - 2 : 0)];
-        const uint32_t ku = s[(lb + ju) * STRIDE + (pu + 1 < 64 ? pu + 1 : 63)];
-        const uint32_t newL = pl >= 2 ? kl + 1u : 0u;
-        const uint32_t newU = pu + 1 < nu ? ku + 1u : KEY_INF;
+        // jl != ju (within one sorted block L <= U)
+        const KeyT* Al = s + (lb + jl) * STRIDE;
+        const KeyT* Au = s + (lb + ju) * STRIDE;
+        auto below = [](int i) { return i < 0 ? 0 : i; };
+        auto above = [](int i) { return i > 63 ? 63 : i; };
+        const uint32_t a2 = Al[below(pl - 2)], a3 = Al[below(pl - 3)], a4 = Al[below(pl - 4)],
+                       a5 = Al[below(pl - 5)];
+        const uint32_t b1 = Au[above(pu + 1)], b2 = Au[above(pu + 2)], b3 = Au[above(pu + 3)],
+                       b4 = Au[above(pu + 4)];
+        const bool s4 = pl >= 4 && pu + 4 <= nu && a4 > b3;
+        const bool s2 = pl >= 2 && pu + 2 <= nu && a2 > b1;
+        const int st = s4 ? 4 : (s2 ? 2 : 1);
+        // jl keeps keys [0, pl - st): new L = key pl - st - 1, new U = key pl - st
+        const uint32_t kl = s4 ? a5 : (s2 ? a3 : a2);
+        const uint32_t newLl = pl - st >= 1 ? kl + 1u : 0u;
+        const uint32_t newUl = s4 ? a4 + 1u : (s2 ? a2 + 1u : lmax);
+        // ju's left part grows to [0, pu + st): new L = key pu + st - 1, new U = key pu + st
+        const uint32_t newLu = s4 ? b3 + 1u : (s2 ? b1 + 1u : umin);
+        const uint32_t ku = s4 ? b4 : (s2 ? b2 : b1);
+        const uint32_t newUu = pu + st < nu ? ku + 1u : KEY_INF;
 #pragma unroll
         for (int j = 0; j < 5; j++) {
             const bool isl = j == jl, isu = j == ju;
-            p[j] += (isu ? 1 : 0) - (isl ? 1 : 0);
-            const uint32_t l2 = isl ? newL : (isu ? umin : L[j]);
-            const uint32_t u2 = isl ? lmax : (isu ? newU : U[j]);
+            p[j] += isu ? st : (isl ? -st : 0);
+            const uint32_t l2 = isl ? newLl : (isu ? newLu : L[j]);
+            const uint32_t u2 = isl ? newUl : (isu ? newUu : U[j]);
             L[j] = l2;
             U[j] = u2;
         }
