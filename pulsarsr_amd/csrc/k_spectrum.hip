@@ -60,6 +60,47 @@ __device__ __forceinline__ void bitonic_sort_regs(uint32_t (&v)[N]) {
             }
 }
 
+// Ascending sort of 64 16-bit keys held two to a register, w[i] = (key i,
+// key i + 32).  The bitonic network is used in its all-ascending form (each
+// merge of two sorted runs of k/2 keys starts by comparing key i with its
+// mirror i ^ (k - 1), then halves with i ^ j, j = k/4 .. 1), so every
+// comparison of the merges up to k = 32 pairs the same half of two registers
+// and one packed 16-bit min/max pair performs two compare-exchanges; only the
+// mirror step of the final merge pairs key a with key 63 - a, i.e. the low half
+// of w[a] with the high half of w[31 - a].
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ce_packed(u16x2& a, u16x2& b) {
+    const u16x2 x = a, y = b;
+    a = __builtin_elementwise_min(x, y);
+    b = __builtin_elementwise_max(x, y);
+}
+__device__ __forceinline__ void sort64_packed(u16x2 (&w)[32]) {
+#pragma unroll
+    for (int k = 2; k <= 32; k <<= 1) {
+#pragma unroll
+        for (int i = 0; i < 32; i++)
+            if ((i ^ (k - 1)) > i) ce_packed(w[i], w[i ^ (k - 1)]);
+#pragma unroll
+        for (int j = k >> 2; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < 32; i++)
+                if ((i ^ j) > i) ce_packed(w[i], w[i ^ j]);
+    }
+#pragma unroll
+    for (int a = 0; a < 16; a++) {  // mirror step of the 64-key merge
+        const int b = 31 - a;
+        const u16x2 sb = w[b].yx;
+        const u16x2 x = __builtin_elementwise_min(w[a], sb), y = __builtin_elementwise_max(w[a], sb);
+        w[a] = u16x2{x.x, y.y};  // keys a, 32 + a
+        w[b] = u16x2{x.y, y.x};  // keys 31 - a, 63 - a
+    }
+#pragma unroll
+    for (int j = 16; j > 0; j >>= 1)
+#pragma unroll
+        for (int i = 0; i < 32; i++)
+            if ((i ^ j) > i) ce_packed(w[i], w[i ^ j]);
+}
+
 constexpr uint32_t SB_BLOCKS = 256;  // natural 64-bin blocks per sorting workgroup (max)
 constexpr uint32_t MB_BLOCKS = 128;  // blocks per median workgroup
 constexpr uint32_t MB_STRIDE = 65;   // LDS words per staged block (odd: spreads banks)
@@ -452,10 +493,24 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
                 ((uint32_t*)raw_nat)[srow + m] = key;
         }
     }
-    if (k_lo == 0 && B == 0) v[0] = KEY_INF;  // DC is not part of any window
-    bitonic_sort_regs<64>(v);
+    if constexpr (F16) {
+        // 16-bit keys two to a register, (key i, key i + 32)
+        u16x2 w[32];
 #pragma unroll
-    for (int i = 0; i < 64; i++) sk[(B + 2) * STRIDE + i] = (KeyT)v[i];  // FP16: KEY_INF keeps its low 16 bits
+        for (int i = 0; i < 32; i++) w[i] = u16x2{(unsigned short)v[i], (unsigned short)v[i + 32]};
+        if (k_lo == 0 && B == 0) w[0].x = 0xffffu;  // DC is not part of any window
+        sort64_packed(w);
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            sk[(B + 2) * STRIDE + i] = w[i].x;
+            sk[(B + 2) * STRIDE + i + 32] = w[i].y;
+        }
+    } else {
+        if (k_lo == 0 && B == 0) v[0] = KEY_INF;  // DC is not part of any window
+        bitonic_sort_regs<64>(v);
+#pragma unroll
+        for (int i = 0; i < 64; i++) sk[(B + 2) * STRIDE + i] = (KeyT)v[i];
+    }
     __syncthreads();  // side keys and the floor are in LDS
     // side block w (waves 0..3): bitonic network across the wave's lanes
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
