@@ -92,17 +92,8 @@ __device__ __forceinline__ void dft<8>(float2* x) {
     x[7] = csub(e3, o3);
 }
 // n = a + 4m, k = b + 4c:  X[b+4c] = DFT4_a(W16^(ab) DFT4_m(x[a+4m])[b])[c]
-template <>
-__device__ __forceinline__ void dft<16>(float2* x) {
-    float2 y[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-        y[a][0] = x[a];
-        y[a][1] = x[a + 4];
-        y[a][2] = x[a + 8];
-        y[a][3] = x[a + 12];
-        dft4(y[a][0], y[a][1], y[a][2], y[a][3]);
-    }
+// (dft16_finish: the twiddles W16^(ab) and the DFT4 over a)
+__device__ __forceinline__ void dft16_finish(float2 (&y)[4][4], float2* x) {
     // W16^j = (cos 2pi j/16, -sin 2pi j/16)
     y[1][1] = cmul(y[1][1], make_float2(C16, -S16));
     y[1][2] = cmul(y[1][2], make_float2(RS2, -RS2));
@@ -122,6 +113,34 @@ __device__ __forceinline__ void dft<16>(float2* x) {
         x[b + 8] = a2;
         x[b + 12] = a3;
     }
+}
+template <>
+__device__ __forceinline__ void dft<16>(float2* x) {
+    float2 y[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        y[a][0] = x[a];
+        y[a][1] = x[a + 4];
+        y[a][2] = x[a + 8];
+        y[a][3] = x[a + 12];
+        dft4(y[a][0], y[a][1], y[a][2], y[a][3]);
+    }
+    dft16_finish(y, x);
+}
+// the same DFT when x[8..15] are zero (zero padding): each first-stage DFT4 of
+// (p, q, 0, 0) is (p + q, p - i q, p - q, p + i q), the values the full network
+// gives, with half the additions
+__device__ __forceinline__ void dft16_lower(float2* x) {
+    float2 y[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const float2 p = x[a], q = x[a + 4];
+        y[a][0] = cadd(p, q);
+        y[a][1] = make_float2(p.x + q.y, p.y - q.x);
+        y[a][2] = csub(p, q);
+        y[a][3] = make_float2(p.x - q.y, p.y + q.x);
+    }
+    dft16_finish(y, x);
 }
 
 // ------------------------------------------------------------- twiddles
@@ -444,9 +463,22 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ 
     const float corr = F16 ? corr16[trial] : 0.f;
     const int t = threadIdx.x, col = t & 15, j = t >> 4;
     float2 v[16];
+    // rows j + 32 m (j < 32) start at sample 2 (c0 + m1 (j + 32 m)).  When the
+    // first sample of row 256 is at or past n, row groups m = 8..15 hold only
+    // padding zeros for the whole workgroup (the zero-padded half of the series
+    // usually does): they are not loaded, and the DFT takes half the adds.
+    const bool upper_empty = 2ull * (c0 + (uint64_t)m1 * 256u) >= (uint64_t)n;  // uniform
 #pragma unroll
-    for (int m = 0; m < 16; m++)
+    for (int m = 0; m < 8; m++)
         v[m] = load_pair<F16>(series, row, 2u * (c0 + col + m1 * (uint32_t)(j + 32 * m)), n, mean, corr);
+    if (upper_empty) {
+#pragma unroll
+        for (int m = 8; m < 16; m++) v[m] = make_float2(0.f, 0.f);
+    } else {
+#pragma unroll
+        for (int m = 8; m < 16; m++)
+            v[m] = load_pair<F16>(series, row, 2u * (c0 + col + m1 * (uint32_t)(j + 32 * m)), n, mean, corr);
+    }
     const float* C512 = cos_tables + costab_offset(512);
     const float2 w1 = twiddle<512>(C512, j), w4 = twiddle<512>(C512, 4 * j);
     const float2* tlo = tw4;
@@ -459,7 +491,10 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ 
         const uint32_t ap = (uint32_t)(((uint64_t)(c0 + (t & 15)) * (16u * (uint32_t)(t >> 4))) & (M - 1));
         P[t] = cmul(thi[ap >> 11], tlo[ap & 2047]);
     }
-    dft<16>(v);
+    if (upper_empty)
+        dft16_lower(v);
+    else
+        dft<16>(v);
     apply_tw<16>(v, w1, w4);
 #pragma unroll
     for (int k = 0; k < 16; k++) buf[j * JP + k * 16 + col] = v[k];
