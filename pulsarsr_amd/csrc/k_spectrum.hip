@@ -713,8 +713,9 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
             offset = interp_offset((double)p0, (double)p1, (double)p2);
             const float pwk = whiten_val(p1, cur.s1, ku, a);
             // j = 1 sums the bin at round(k + offset): offset is in [-0.5, 0.5] and
-            // round() takes halves away from zero, so that is k or k + 1 (<= nbins - 1)
-            const bool up = round((double)ku + offset) > (double)ku;
+            // round() takes halves away from zero, so that is k or k + 1 (<= nbins - 1),
+            // and k + 1 exactly when the rounded sum reaches k + 0.5
+            const bool up = (double)ku + offset >= (double)ku + 0.5;
             const float w1 = up ? whiten_val(p2, cur.s2, ku + 1, a) : pwk;
             S1 = (double)w1;
             sc[0][tid] = S1 - 1.0;  // / sqrt(1): exact
