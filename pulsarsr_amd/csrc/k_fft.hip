@@ -338,11 +338,13 @@ __device__ double block_sum(double v, double* red) {  // red: blockDim/64 entrie
 // Power of X[k] from Z[k] and Z[M-k] (zp) with W = W_N^k:
 //   X = (Z + conj Zp)/2 + W (Z - conj Zp)/(2i)
 __device__ __forceinline__ float r2c_power(float2 zk, float2 zp, float2 w) {
-    const float2 e = make_float2(0.5f * (zk.x + zp.x), 0.5f * (zk.y - zp.y));
-    const float2 od = make_float2(0.5f * (zk.y + zp.y), -0.5f * (zk.x - zp.x));
+    // e, od and x are twice E[k], O[k] and X[k]; scaling by powers of two is
+    // exact, so 0.25 |2 X|^2 is the same float as |X|^2 from the halved terms
+    const float2 e = make_float2(zk.x + zp.x, zk.y - zp.y);
+    const float2 od = make_float2(zk.y + zp.y, zp.x - zk.x);
     const float2 x = cadd(e, cmul(w, od));
     const float xx = x.x * x.x, yy = x.y * x.y;
-    return xx + yy;
+    return 0.25f * (xx + yy);
 }
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
