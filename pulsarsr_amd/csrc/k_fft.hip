@@ -239,6 +239,15 @@ __device__ __forceinline__ void apply_tw(float2* x, float2 w1, float2 w4) {
     }
 }
 
+// x[k] *= tw[k * S], k = 1 .. 15, with the powers read from an LDS table whose
+// rows are the powers k (the threads of a wave that read different columns hit
+// consecutive words: no bank conflicts)
+template <int S>
+__device__ __forceinline__ void apply_tw_col(float2* x, const float2* tw) {
+#pragma unroll
+    for (int k = 1; k < 16; k++) x[k] = cmul(x[k], tw[k * S]);
+}
+
 constexpr int FFT_THREADS = 512;
 
 template <int L, int V, bool PAD, int R, int NS>
@@ -664,6 +673,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
                                                     const float* __restrict__ cos_tables) {
     constexpr int M1 = 4096, JP = 272, RS = 16 * JP;
     __shared__ float2 buf[2 * RS];
+    __shared__ float2 TA[16 * 16];  // W_256^(a k) at k * 16 + a
     __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.y;
     const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
@@ -675,6 +685,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
         const float2* src = w + (size_t)(row ? r2 : r1) * M1 + tt;
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = src[256 * m];
+        if (t < 256) TA[t] = twiddle<256>(cos_tables + costab_offset(256), (t >> 4) * (t & 15));
         const float* C4096 = cos_tables + costab_offset(4096);
         const float2 w1 = twiddle<4096>(C4096, tt), w4 = twiddle<4096>(C4096, 4 * tt);
         dft<16>(v);
@@ -687,10 +698,8 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
         const int row = t >> 8, a = t & 15, k1 = (t >> 4) & 15;
 #pragma unroll
         for (int b = 0; b < 16; b++) v[b] = buf[row * RS + k1 * JP + b * 16 + a];
-        const float* C256 = cos_tables + costab_offset(256);
-        const float2 w1 = twiddle<256>(C256, a), w4 = twiddle<256>(C256, 4 * a);
         dft<16>(v);
-        apply_tw<16>(v, w1, w4);
+        apply_tw_col<16>(v, TA + a);  // written before the previous barrier
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 16; q++) buf[row * RS + q * JP + k1 * 16 + (a ^ k1)] = v[q];
