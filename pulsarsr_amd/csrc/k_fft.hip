@@ -467,6 +467,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ 
     constexpr int C = 16, JP = 272;  // exchange rows of 256 + 16 (bank offset between rows)
     __shared__ float2 buf[32 * JP];
     __shared__ float2 P[256];  // W_M^(16 (c0 + col) q1) at q1 * 16 + col
+    __shared__ float2 TJ[16 * 32];  // W_512^(j k) at k * 32 + j
     const uint2 wb = xcd_block();
     const uint32_t trial = wb.y, c0 = wb.x * C;
     const size_t row = (size_t)trial * series_pitch;
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ 
             v[m] = load_pair<F16>(series, row, 2u * (c0 + col + m1 * (uint32_t)(j + 32 * m)), n, mean, corr);
     }
     const float* C512 = cos_tables + costab_offset(512);
-    const float2 w1 = twiddle<512>(C512, j), w4 = twiddle<512>(C512, 4 * j);
+    TJ[t] = twiddle<512>(C512, (t >> 5) * (t & 31));  // one table entry per thread
     const float2* tlo = tw4;
     const float2* thi = tw4 + 2048;
     const int j0 = (t >> 4) & 1, k1 = t >> 5;
@@ -506,7 +507,8 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ 
         dft16_lower(v);
     else
         dft<16>(v);
-    apply_tw<16>(v, w1, w4);
+    __syncthreads();  // TJ
+    apply_tw_col<32>(v, TJ + j);
 #pragma unroll
     for (int k = 0; k < 16; k++) buf[j * JP + k * 16 + col] = v[k];
     __syncthreads();
