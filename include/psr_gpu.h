@@ -188,7 +188,10 @@ int psr_last_stage_times(const psr_ctx *ctx, psr_stage_times *out);
 int psr_periodicity(psr_ctx *ctx, const float *series, size_t n, double tsamp, double dm,
                     const psr_search_cfg *cfg, psr_candidate *out, size_t cap, size_t *count);
 /* The spectral stages of periodicity for checks: raw |X|^2 and whitened power
- * (crates/search/src/lib.rs:35-72), each nbins = next_pow2(n)+1 floats. */
+ * (crates/search/src/lib.rs:35-72), each nbins = next_pow2(n)+1 floats.  With
+ * cfg->precision = PSR_PRECISION_FP16 the series (less its mean) is stored in
+ * half precision and the spectrum as FP16 amplitudes, as psr_search stores
+ * them; raw then holds the power a * a the whitening and scoring use. */
 int psr_spectrum(psr_ctx *ctx, const float *series, size_t n, double tsamp,
                  const psr_search_cfg *cfg, float *raw, float *white, size_t *nbins);
 

@@ -877,12 +877,21 @@ int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp,
     *nbins = g.nbins;
     if (!raw && !white) return PSR_OK;
     psr_search_cfg cf = *cfg;
-    cf.precision = PSR_PRECISION_FP32;
+    const bool f16 = cfg->precision == PSR_PRECISION_FP16;
+    if (!f16) cf.precision = PSR_PRECISION_FP32;
     cf.threshold = INFINITY;  // no peaks needed
     SearchParams p;
     std::vector<uint32_t> birdies;
     if ((rc = make_params(g, tsamp, &cf, &p, &birdies))) return rc;
     if ((rc = upload_series(c, series, n, g))) return rc;
+    if (f16) {  // FP16 storage: the mean-subtracted series rounded to half precision
+        double mean = 0.0;
+        for (size_t i = 0; i < n; i++) mean += (double)series[i];
+        mean /= (double)n;
+        std::vector<_Float16> h(n);
+        for (size_t i = 0; i < n; i++) h[i] = (_Float16)(float)((double)series[i] - mean);
+        HIP_TRY(hipMemcpy(c->series.p, h.data(), n * sizeof(_Float16), hipMemcpyHostToDevice));
+    }
     const double zero = 0.0;
     HIP_TRY(hipMemcpy(c->dms.p, &zero, sizeof(double), hipMemcpyHostToDevice));
     psr_candidate dummy[1];
@@ -891,7 +900,13 @@ int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp,
         return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::vector<float> r(g.nbins), sc(g.nblocks);
-    HIP_TRY(hipMemcpy(r.data(), c->raw_nat.p, g.nbins * sizeof(float), hipMemcpyDeviceToHost));
+    if (f16) {  // stored amplitudes a; the search's power is a * a in f32
+        std::vector<_Float16> a(g.nbins);
+        HIP_TRY(hipMemcpy(a.data(), c->raw_nat.p, g.nbins * sizeof(_Float16), hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < g.nbins; k++) r[k] = (float)a[k] * (float)a[k];
+    } else {
+        HIP_TRY(hipMemcpy(r.data(), c->raw_nat.p, g.nbins * sizeof(float), hipMemcpyDeviceToHost));
+    }
     HIP_TRY(hipMemcpy(sc.data(), c->scales.p, g.nblocks * sizeof(float), hipMemcpyDeviceToHost));
     if (raw) std::memcpy(raw, r.data(), g.nbins * sizeof(float));
     if (white) {

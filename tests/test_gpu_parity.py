@@ -150,6 +150,35 @@ def test_spectrum_matches_oracle(ctx, n):
     assert np.linalg.norm(white - owhite) <= 1e-5 * np.linalg.norm(owhite)
 
 
+@pytest.mark.parametrize("n", [263500, 1500000, 2097152])
+def test_fp16_whitening_scales_are_exact_window_medians(ctx, n):
+    # FP16 storage (packed 16-bit block sort, windowed cut search): every 64-bin
+    # block's scale is max(max(upper median / ln2, floor), 1e-20) of the stored
+    # powers a * a over bins [64b - 128, 64b + 192) clipped to [1, nbins)
+    # (crates/search/src/lib.rs:45-60), so the whitened power must equal
+    # raw / scale recomputed from the returned raw spectrum with the oracle's
+    # median, bit for bit (the floor, mean * 1e-6, is far below every median here)
+    print("fp16 whitening n=%d" % n, flush=True)
+    rng = np.random.default_rng(n + 7)
+    t = np.arange(n)
+    x = (rng.standard_normal(n) * 16 + 2000 + 30 * np.sin(2 * np.pi * t / 97.3)).round().astype(np.float32)
+    raw, white = search.spectrum(x, 64e-6, search.Config(min_hz=10.0, precision=PRECISION_FP16))
+    nb = len(raw)
+    assert np.all(np.isfinite(raw)) and raw[1:].min() >= 0.0
+    floor = np.float32(raw.astype(np.float64).sum() / nb * 1e-6)
+    ln2 = np.float32(math.log(2.0))
+    exp = np.empty(nb, np.float32)
+    for b in range((nb + 63) // 64):
+        lo, hi = max(64 * b - 128, 1), min(64 * b + 192, nb)
+        med = np.float32(oracle.median(raw[lo:hi]))
+        assert med / ln2 > floor
+        scale = max(max(med / ln2, floor), np.float32(1e-20))
+        exp[64 * b:64 * b + 64] = raw[64 * b:64 * b + 64] / scale
+    exp[0] = 0.0
+    bad = np.nonzero(white != exp)[0]
+    assert bad.size == 0, (bad[:8], white[bad[:8]], exp[bad[:8]])
+
+
 def match_lists(got, ref, threshold, max_candidates=32):
     """Per-trial candidate lists agree up to threshold-edge flips; returns worst rel score error."""
     worst = 0.0
