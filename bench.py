@@ -138,8 +138,9 @@ def main():
                     help="first copy this many MiB device-to-device (HBM counter calibration for rocprofv3)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
-              "(DM trials would shard across GPUs; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1): the DM trials of a multi-GPU run would be "
+              "split into contiguous ranges, one process per GPU, each running this one-GPU workload",
+              file=sys.stderr)
         sys.exit(2)
 
     from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, synth
