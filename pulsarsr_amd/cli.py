@@ -3,8 +3,8 @@ dedisperse, header.  Run as `python -m pulsarsr_amd <command> ...`.
 
 Arguments, defaults, validation messages, console lines and the JSON outputs
 (header.json, report.json schema 2, fold.json, series.json/series.f32) follow
-the reference; the PNG plots and the `inspect`/`synth` commands are not part
-of this build.  Every stage runs on the GPU through libpsrgpu: packed ingest,
+the reference; the PNG plots and the `inspect`/`synth` commands are not
+provided.  Every stage runs on the GPU through libpsrgpu: packed ingest,
 rfi::mask, the DM-trial search, confirmation::analyze, dedispersion and folds.
 """
 import argparse
