@@ -112,8 +112,11 @@ __host__ __device__ constexpr int costab_offset(int L) {
 }
 constexpr int COSTAB_SIZE = costab_offset(2 * COSTAB_MAX);
 void fill_costabs(float* host_table);
-// Trials per four-step FFT batch (the work buffer holds this many trials).
-constexpr uint64_t FFT_BATCH_BYTES = 1ull << 30;
+// Intermediate bytes per four-step FFT batch (the work buffer holds this many
+// trials' intermediates): 192 MiB (11 trials at 2^22) stays within the 256 MiB
+// Infinity Cache between the two passes; measured against 128 MiB to 1 GiB
+// batches it gives the shortest headline step.
+constexpr uint64_t FFT_BATCH_BYTES = 192ull << 20;
 uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials);
 // whitening scales + natural-order raw spectrum; `sorted` (spectrum-sized) and
 // `floors` (one float per trial) are scratch, unused when whiten_fused(g)
