@@ -186,7 +186,7 @@ constexpr int DD_TILE = 512;
 constexpr int DD_TRIALS = DD_WAVES * DD_M;
 static_assert(DD_TRIALS == (int)DEDISP_BLOCK_TRIALS, "block size shared with the host");
 constexpr int DD_XCDS = 8;
-constexpr int DD_AHEAD = 4;  // trials whose LDS reads are issued ahead of their adds
+constexpr int DD_AHEAD = 2;  // trials whose LDS reads are issued ahead of their adds
 constexpr int DD_ROW_BYTES = 48 * 1024;  // rows of one stage (at most; two workgroups per CU up to 2048 channels)
 constexpr int DD_RAW_BYTES = 20 * 1024;  // raw byte windows of one stage (at most)
 constexpr int DD_RSRC_WORD3 = 0x00020000;  // buffer descriptor word 3: raw 32-bit data

@@ -632,7 +632,7 @@ constexpr int HS_T = 256;       // threads: scores for HS_T bins
 constexpr int HS_OUT = HS_T - 2;  // peak tests per workgroup
 constexpr int HS_NJ = 15;         // harmonics j = 2 .. 16
 constexpr int HS_LIST = 64;       // listed bins summed per round
-constexpr int HS_TILES = 4;       // tiles of HS_OUT bins per workgroup, the next one's inputs prefetched
+constexpr int HS_TILES = 2;       // tiles of HS_OUT bins per workgroup, the next one's inputs prefetched
 
 // One workgroup scores HS_TILES tiles of HS_T consecutive bins in turn (tiles
 // overlap by two bins, the peak test's neighbours), loading the next tile's
