@@ -76,11 +76,11 @@ def top_families(cands, n, tsamp, k=20):
 # calibrated against a 1 GiB device copy), from the rocprofv3 passes recorded in
 # profiles/hbm_traffic_headline-1024ch-8bit.md (one 768-trial launch).
 MEASURED_TRAFFIC_BYTES_PER_UNIT = {
-    "k_dedisp_u8": 7.74,
-    "k_fft_pass1r": 10.02,
-    "k_fft_pass2r": 10.03,
+    "k_dedisp_u8": 7.59,
+    "k_fft_pass1r": 10.03,
+    "k_fft_pass2r": 10.04,
     "k_whiten_fused": 4.08,
-    "k_harmsum": 1.83,
+    "k_harmsum": 1.82,
 }
 
 
