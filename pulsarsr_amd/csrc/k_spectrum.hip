@@ -447,11 +447,7 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
     KeyT* sk = (KeyT*)lds_w;             // sorted blocks, local block lb = global - b0 + 2
     uint32_t* side = lds_w + MAIN / 4;  // [4][64]: blocks b0-2, b0-1, b0+SB, b0+SB+1
     __shared__ float sfloor;
-    // FP16 workgroups (four per CU) in dispatch order; FP32 ones (two per CU,
-    // twice the bytes per line) walk XCD-contiguous runs so that the workgroups
-    // sharing a column line meet in one L2.  Both measured faster than the other
-    // choice.
-    const uint2 wb = F16 ? make_uint2(blockIdx.x, blockIdx.y) : xcd_block();
+    const uint2 wb = xcd_block();
     const uint32_t trial = wb.y;
     const uint32_t k1a = wb.x * cw;
     const uint32_t k_lo = k1a * m2, b0 = k_lo / 64;
@@ -661,8 +657,7 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
     __shared__ uint8_t lbin[HS_T];  // their thread index
     __shared__ float hv[HS_LIST][HS_NJ + 2];  // odd row stride: conflict-free column reads
     __shared__ uint32_t nlist;
-    // dispatch order (measured faster than XCD-contiguous runs of tiles)
-    const uint2 wb = make_uint2(blockIdx.x, blockIdx.y);
+    const uint2 wb = xcd_block();  // neighbouring bins' harmonics share lines on one XCD
     const uint32_t trial = wb.y;
     const size_t row = (size_t)trial * spec_pitch;
     const float* scl = scales + (size_t)trial * nblocks;
