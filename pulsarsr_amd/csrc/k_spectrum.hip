@@ -535,6 +535,116 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
             window_scale<F16, KeyT, STRIDE>(sk, SB_BLOCKS, nsorted, nblocks, nbins, fl);
 }
 
+// FP16 form of k_whiten_fused for two column groups per workgroup: the keys
+// stay packed two to a register, (key i, key i + 32), from the loads on, and
+// the second group's 64 loads per lane are issued as soon as the first group's
+// keys are in the sorting registers, so their latency overlaps the first
+// group's sort, LDS exchange and window medians.
+constexpr uint32_t WH_GROUPS = 2;
+__global__ __launch_bounds__(256) void k_whiten_fused16(
+    const uint16_t* __restrict__ spec_t, uint32_t m, uint32_t m1, uint32_t m2, uint32_t cw,
+    uint32_t nbins, uint32_t nblocks, uint64_t spec_pitch, const double* __restrict__ partial,
+    uint32_t npart, uint16_t* __restrict__ raw_nat, float* __restrict__ scales) {
+    constexpr int STRIDE = wf_stride<true>();
+    constexpr size_t MAIN = wf_lds_bytes<true>() - 4 * 64 * 4;
+    extern __shared__ __align__(16) uint32_t lds_w[];
+    uint16_t* sk = (uint16_t*)lds_w;     // sorted blocks, local block lb = global - b0 + 2
+    uint32_t* side = lds_w + MAIN / 4;  // [4][64]: blocks b0-2, b0-1, b0+SB, b0+SB+1
+    __shared__ float sfloor;
+    const uint2 wb = xcd_block();
+    const uint32_t trial = wb.y;
+    const uint32_t ngroups = m1 / cw;
+    const uint32_t nsorted = (m + 63) / 64;
+    const uint32_t cpb = m2 / 64;  // blocks per column
+    const size_t srow = (size_t)trial * spec_pitch;
+    const uint32_t col = threadIdx.x % cw, ch = threadIdx.x / cw;
+    const uint32_t B = ch + cpb * col;  // local natural block
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // this lane's block of group gi: column gi cw + col, rows 64 ch .. 64 ch + 63
+    // (one register per key while the loads are in flight: packing them at once
+    // would wait for them)
+    auto load_group = [&](uint32_t gi, uint32_t (&r)[64]) {
+        const uint16_t* src = spec_t + srow + (size_t)(64 * ch) * m1 + gi * cw + col;
+#pragma unroll
+        for (int i = 0; i < 64; i++) r[i] = src[(size_t)i * m1];
+    };
+    if (threadIdx.x == 0) {
+        double S = 0.0;
+        for (uint32_t i = 0; i < npart; i++) S += partial[(size_t)trial * npart + i];
+        sfloor = (float)(S / (double)nbins * 1e-6);
+    }
+    uint32_t pre[64];
+    load_group(wb.x * WH_GROUPS, pre);
+    for (uint32_t it = 0; it < WH_GROUPS; it++) {
+        const uint32_t gi = wb.x * WH_GROUPS + it;
+        if (gi >= ngroups) break;
+        u16x2 w[32];
+#pragma unroll
+        for (int i = 0; i < 32; i++) w[i] = u16x2{(unsigned short)pre[i], (unsigned short)pre[i + 32]};
+        const uint32_t k1a = gi * cw, k_lo = k1a * m2, b0 = k_lo / 64;
+        {  // side blocks: thread t takes element t % 64 of side block t / 64
+            const uint32_t sb = threadIdx.x >> 6, i = threadIdx.x & 63;
+            const int64_t gb = sb < 2 ? (int64_t)b0 - 2 + sb : (int64_t)b0 + SB_BLOCKS + (sb - 2);
+            uint32_t key = KEY_INF;
+            if (gb >= 0 && gb < (int64_t)nsorted) {
+                const uint32_t K = (uint32_t)gb * 64 + i;
+                if (K > 0 && K < m) key = spec_t[srow + (size_t)(K % m2) * m1 + K / m2];
+            } else if (gb == (int64_t)nsorted && gb < (int64_t)nblocks && i == 0) {
+                key = spec_t[srow + m];  // block nsorted holds Nyquist alone
+            }
+            side[threadIdx.x] = key;
+        }
+        {  // natural-order raw spectrum: keys 8c .. 8c + 7 of the block
+            uint16_t* o = raw_nat + srow + (size_t)(b0 + B) * 64;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int h = c >> 2, i0 = 8 * (c & 3);
+                uint32_t q[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    q[e] = h ? (uint32_t)w[i0 + 2 * e].y | (uint32_t)w[i0 + 2 * e + 1].y << 16
+                             : (uint32_t)w[i0 + 2 * e].x | (uint32_t)w[i0 + 2 * e + 1].x << 16;
+                *(uint4*)(o + 8 * c) = make_uint4(q[0], q[1], q[2], q[3]);
+            }
+            if (k1a + cw == m1 && threadIdx.x == 0) raw_nat[srow + m] = spec_t[srow + m];  // Nyquist
+        }
+        if (k_lo == 0 && B == 0) w[0].x = 0xffffu;  // DC is not part of any window
+        u16x2 v[32];
+#pragma unroll
+        for (int i = 0; i < 32; i++) v[i] = w[i];
+        if (it + 1 < WH_GROUPS && gi + 1 < ngroups) load_group(gi + 1, pre);  // in flight from here
+        sort64_packed(v);
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            sk[(B + 2) * STRIDE + i] = v[i].x;
+            sk[(B + 2) * STRIDE + i + 32] = v[i].y;
+        }
+        __syncthreads();  // side keys and the floor are in LDS
+        // side block w (waves 0..3): bitonic network across the wave's lanes
+        uint32_t x = side[threadIdx.x];
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t pv = (uint32_t)__shfl_xor((int)x, j);
+                const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+                x = lower == up ? min(x, pv) : max(x, pv);
+            }
+        sk[(wave < 2 ? wave : SB_BLOCKS + wave) * STRIDE + lane] = (uint16_t)x;
+        if (threadIdx.x < 2 * STRIDE)  // local blocks SB + 4 and SB + 5: never counted, kept defined
+            sk[(SB_BLOCKS + 4) * STRIDE + threadIdx.x] = (uint16_t)KEY_INF;
+        __syncthreads();
+        const float fl = sfloor;
+        const uint32_t bm = threadIdx.x;  // medians in natural block order
+        scales[(size_t)trial * nblocks + b0 + bm] =
+            window_scale<true, uint16_t, STRIDE>(sk, bm, b0 + bm, nblocks, nbins, fl);
+        if (bm == 0 && k1a + cw == m1 && nsorted < nblocks)  // the Nyquist block
+            scales[(size_t)trial * nblocks + nsorted] =
+                window_scale<true, uint16_t, STRIDE>(sk, SB_BLOCKS, nsorted, nblocks, nbins, fl);
+        __syncthreads();  // the next group rewrites the sorted blocks and side keys
+    }
+}
+
 static uint32_t sort_columns(const SpecGeom& g) {  // columns per sorting workgroup
     uint32_t cw = g.m1;
     while (cw > 1 && (uint64_t)cw * g.m2 > (uint64_t)SB_BLOCKS * 64) cw >>= 1;
@@ -557,10 +667,12 @@ void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp1
         if (fp16) {
             constexpr size_t shm = wf_lds_bytes<true>();
             if (shm > 64 * 1024)
-                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_whiten_fused<true>),
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_whiten_fused16),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-            hipLaunchKernelGGL(k_whiten_fused<true>, gs, dim3(256), shm, s, spec_t, g.m, g.m1, g.m2, cw, g.nbins,
-                               g.nblocks, g.spec_pitch, partial, npartials, raw_nat, scales);
+            const dim3 gh((g.m1 / cw + WH_GROUPS - 1) / WH_GROUPS, ntrials);
+            hipLaunchKernelGGL(k_whiten_fused16, gh, dim3(256), shm, s, (const uint16_t*)spec_t, g.m, g.m1,
+                               g.m2, cw, g.nbins, g.nblocks, g.spec_pitch, partial, npartials,
+                               (uint16_t*)raw_nat, scales);
         } else {
             constexpr size_t shm = wf_lds_bytes<false>();
             if (shm > 64 * 1024)
