@@ -76,10 +76,10 @@ def top_families(cands, n, tsamp, k=20):
 # calibrated against a 1 GiB device copy), from the rocprofv3 passes recorded in
 # profiles/hbm_traffic_headline-1024ch-8bit.md (one 768-trial launch).
 MEASURED_TRAFFIC_BYTES_PER_UNIT = {
-    "k_dedisp_u8": 7.59,
+    "k_dedisp_u8": 7.53,
     "k_fft_pass1r": 10.03,
     "k_fft_pass2r": 10.04,
-    "k_whiten_fused": 4.08,
+    "k_whiten_fused16": 5.89,
     "k_harmsum": 1.82,
 }
 
@@ -216,7 +216,7 @@ def main():
     r = (stages["fft_len"] // 2 + 1) / n
     spec_ms = stages["fft_ms"] + stages["whiten_ms"] + stages["harmsum_ms"]
     spec_alg = 2 + 2 * r + 2 * r + 2 * r
-    spec_traffic = sum(tpu.get(k, 0.0) for k in ("k_fft_pass1r", "k_fft_pass2r", "k_whiten_fused", "k_harmsum")) or None
+    spec_traffic = sum(tpu.get(k, 0.0) for k in ("k_fft_pass1r", "k_fft_pass2r", "k_whiten_fused16", "k_harmsum")) or None
     res = {
         "metric": "DM-trial·samples/s (dedisp+FFT+harmsum), FP16-store vs FP32 at same candidates",
         "value": units / (ms16 * 1e-3),
