@@ -170,9 +170,23 @@ int psr_obs_mask(psr_ctx *ctx, psr_obs *obs, uint64_t block_samples, float thres
 /* ---- the hot path ------------------------------------------------------- */
 /* dedisperse::delays + dedisperse::apply for `ntrials` DMs
  * (crates/dedisperse/src/lib.rs:9-34, 62-90).  length==0 means None (full
- * available length).  out[ntrials*length] f32; *out_len gets the length used. */
+ * available length).  out[ntrials*length] f32; *out_len gets the length used
+ * (out may be NULL to query it).  Lengths up to 2^31 samples. */
 int psr_dedisperse(psr_ctx *ctx, psr_obs *obs, const double *dms, size_t ntrials, size_t length,
                    float *out, size_t *out_len);
+/* dedisperse::apply(data, nsamples, delays, length) (crates/dedisperse/src/lib.rs:62-90),
+ * one to one: `data` is channels [ch_lo, ch_hi) of the device observation (the
+ * subband slices of crates/search/src/confirmation.rs:362-372 are such ranges;
+ * the whole band is 0..nchans) and delays[ch_hi - ch_lo] is the caller's
+ * per-channel delay table in samples (any values, e.g. the all-zero tables of
+ * confirmation.rs:253,321).  length==0 means None.  Sums run in channel order in
+ * f32, as the reference adds them.  Errors as apply(): an empty or out-of-range
+ * channel range is "array shape mismatch" (PSR_E_SHAPE), a delay past the
+ * observation "delay exceeds observation" (PSR_E_INVALID), a length outside
+ * [2, nsamples - max delay] "invalid output length" (PSR_E_SHAPE).  out[length]
+ * f32, or NULL to query *out_len. */
+int psr_apply(psr_ctx *ctx, psr_obs *obs, const uint64_t *delays, uint32_t ch_lo, uint32_t ch_hi,
+              size_t length, float *out, size_t *out_len);
 
 /* The per-trial loop of crates/cli/src/main.rs:462-469: for every DM, delays ->
  * apply(Some(length)) -> search::periodicity (crates/search/src/lib.rs:33-145).
@@ -256,6 +270,12 @@ typedef struct {
 } psr_outcome;
 
 void psr_rules_default(psr_rules *out);
+/* confirmation::measure (confirmation.rs:107-141): the fixed on/off window
+ * measurement of x[n], sample i at phase bin of (i + offset) * tsamp / period
+ * (confirmation.rs:103-105).  Host code: sequential f64 sums in the
+ * reference's order.  window->bins must be positive. */
+int psr_measure(const float *x, size_t n, uint64_t offset, double tsamp, double period,
+                const psr_window *window, psr_measurement *out);
 /* confirmation::group (confirmation.rs:177-199): families of harmonically
  * related candidates, strongest first.  leaders[limit]; members[n] lists
  * (family, input index) in the order members were appended. */

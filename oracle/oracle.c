@@ -80,7 +80,7 @@ float oracle_median(const float *v, size_t n) {
     free(k);
     return r;
 }
-static double median64(const double *v, size_t n) {
+double oracle_median64(const double *v, size_t n) {
     if (n == 0) return 0.0;
     int64_t *k = (int64_t *)malloc(n * sizeof(int64_t));
     for (size_t i = 0; i < n; i++) k[i] = total_key64(v[i]);
@@ -511,10 +511,10 @@ size_t oracle_periodicity(const float *series, size_t n, double tsamp, double dm
     return r;
 }
 
-/* All-f64 twin: same algorithm, double everywhere (the FP64 oracle). */
-size_t oracle_periodicity_f64(const float *series, size_t n, double tsamp, double dm,
-                              const oracle_config *cfg, oracle_candidate *out, size_t cap) {
-    if (n < 2 || !isfinite(tsamp) || tsamp <= 0.0) return 0;
+/* All-f64 twin of oracle_spectrum: crates/search/src/lib.rs:35-72 with every
+ * f32 step in double (the FP64 oracle's spectrum stages). */
+size_t oracle_spectrum_f64(const float *series, size_t n, double tsamp, const oracle_config *cfg,
+                           double *raw, double *pw) {
     size_t size = next_pow2(n) * 2, nb = size / 2 + 1;
     double mean = 0.0;
     for (size_t i = 0; i < n; i++) mean += (double)series[i];
@@ -524,7 +524,6 @@ size_t oracle_periodicity_f64(const float *series, size_t n, double tsamp, doubl
     double *re = (double *)malloc(nb * sizeof(double)), *im = (double *)malloc(nb * sizeof(double));
     oracle_rfft64(x, size, re, im);
     free(x);
-    double *raw = (double *)malloc(nb * sizeof(double)), *pw = (double *)malloc(nb * sizeof(double));
     double total = 0.0;
     for (size_t k = 0; k < nb; k++) {
         raw[k] = re[k] * re[k] + im[k] * im[k];
@@ -539,7 +538,7 @@ size_t oracle_periodicity_f64(const float *series, size_t n, double tsamp, doubl
         size_t lo = center >= 128 ? center - 128 : 0;
         if (lo < 1) lo = 1;
         size_t hi = center + 192 < nb ? center + 192 : nb;
-        double med = median64(raw + lo, hi > lo ? hi - lo : 0);
+        double med = oracle_median64(raw + lo, hi > lo ? hi - lo : 0);
         double scale = fmax(fmax(med / M_LN2, floor64), 1e-20);
         size_t end = center + 64 < nb ? center + 64 : nb;
         for (size_t k = center; k < end; k++) pw[k] /= scale;
@@ -554,6 +553,16 @@ size_t oracle_periodicity_f64(const float *series, size_t n, double tsamp, doubl
                 break;
             }
     }
+    return size;
+}
+
+/* All-f64 twin: same algorithm, double everywhere (the FP64 oracle). */
+size_t oracle_periodicity_f64(const float *series, size_t n, double tsamp, double dm,
+                              const oracle_config *cfg, oracle_candidate *out, size_t cap) {
+    if (n < 2 || !isfinite(tsamp) || tsamp <= 0.0) return 0;
+    size_t size = next_pow2(n) * 2, nb = size / 2 + 1;
+    double *raw = (double *)malloc(nb * sizeof(double)), *pw = (double *)malloc(nb * sizeof(double));
+    oracle_spectrum_f64(series, n, tsamp, cfg, raw, pw);
     size_t r = harmonic_search(raw, pw, nb, size, n, tsamp, dm, cfg, out, cap);
     free(raw);
     free(pw);
@@ -701,6 +710,8 @@ typedef struct {
     const oracle_config *cfg;
     oracle_candidate *out;
     uint32_t *counts;
+    float *series_out; /* optional [ntrials][length] */
+    int f64;
     size_t ntrials;
     size_t next;
     int err;
@@ -722,8 +733,9 @@ static void *trial_worker(void *arg) {
             j->err = 1;
             continue;
         }
-        j->counts[t] = (uint32_t)oracle_periodicity(x, got, j->tsamp, j->dms[t], j->cfg,
-                                                     j->out + t * j->cap, j->cap);
+        if (j->series_out) memcpy(j->series_out + t * j->length, x, got * sizeof(float));
+        j->counts[t] = (uint32_t)(j->f64 ? oracle_periodicity_f64 : oracle_periodicity)(
+            x, got, j->tsamp, j->dms[t], j->cfg, j->out + t * j->cap, j->cap);
     }
     free(d);
     free(x);
@@ -735,7 +747,17 @@ int oracle_search_trials(const float *data, size_t nsamples, const double *freqs
                          double tsamp, const double *dms, size_t ntrials, size_t length,
                          const oracle_config *cfg, oracle_candidate *out, size_t cap,
                          uint32_t *counts, int threads) {
+    return oracle_search_trials_ex(data, nsamples, freqs, nch, tsamp, dms, ntrials, length, cfg, out,
+                                   cap, counts, NULL, 0, threads);
+}
+
+int oracle_search_trials_ex(const float *data, size_t nsamples, const double *freqs, size_t nch,
+                            double tsamp, const double *dms, size_t ntrials, size_t length,
+                            const oracle_config *cfg, oracle_candidate *out, size_t cap,
+                            uint32_t *counts, float *series_out, int f64, int threads) {
     trial_job j;
+    j.series_out = series_out;
+    j.f64 = f64;
     j.data = data;
     j.nsamples = nsamples;
     j.nch = nch;

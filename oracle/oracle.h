@@ -61,6 +61,8 @@ int oracle_decode(const uint8_t *payload, size_t payload_len, int nbits, size_t 
                   size_t limit, float *out, size_t *nsamples_out);
 /* crates/rfi/src/lib.rs:4-12 (upper median, total order) */
 float oracle_median(const float *v, size_t n);
+/* The same upper median over doubles (the FP64 oracle's whitening). */
+double oracle_median64(const double *v, size_t n);
 /* crates/rfi/src/lib.rs:31-96.  Mutates data; channels/blocks written as 0/1 flags. */
 double oracle_mask(float *data, size_t nsamples, size_t nch, size_t block_samples, float threshold,
                    uint8_t *bad_channels, uint8_t *bad_blocks);
@@ -75,6 +77,9 @@ size_t oracle_spectrum(const float *series, size_t n, double tsamp, const oracle
 /* crates/search/src/lib.rs:33-145.  Returns the number of candidates written. */
 size_t oracle_periodicity(const float *series, size_t n, double tsamp, double dm,
                           const oracle_config *cfg, oracle_candidate *out, size_t cap);
+/* All-f64 twin of oracle_spectrum (double raw power and whitened power). Returns N_fft. */
+size_t oracle_spectrum_f64(const float *series, size_t n, double tsamp, const oracle_config *cfg,
+                           double *raw, double *white);
 /* All-f64 twin of oracle_periodicity (FP64 oracle for reduced-precision error reports). */
 size_t oracle_periodicity_f64(const float *series, size_t n, double tsamp, double dm,
                               const oracle_config *cfg, oracle_candidate *out, size_t cap);
@@ -156,6 +161,12 @@ int oracle_search_trials(const float *data, size_t nsamples, const double *freqs
                          double tsamp, const double *dms, size_t ntrials, size_t length,
                          const oracle_config *cfg, oracle_candidate *out, size_t cap,
                          uint32_t *counts, int threads);
+/* The same loop; series_out (optional) receives every trial's dedispersed series
+ * [ntrials][length]; f64 != 0 searches with oracle_periodicity_f64. */
+int oracle_search_trials_ex(const float *data, size_t nsamples, const double *freqs, size_t nch,
+                            double tsamp, const double *dms, size_t ntrials, size_t length,
+                            const oracle_config *cfg, oracle_candidate *out, size_t cap,
+                            uint32_t *counts, float *series_out, int f64, int threads);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,11 @@ def lib():
             "oracle_refine": (C.c_double, [fp, C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_size_t, dp, dp, fp, up]),
             "oracle_search_trials": (C.c_int, [fp, C.c_size_t, dp, C.c_size_t, C.c_double, dp, C.c_size_t, C.c_size_t,
                                                C.POINTER(OConfig), C.POINTER(OCand), C.c_size_t, C.POINTER(C.c_uint32), C.c_int]),
+            "oracle_search_trials_ex": (C.c_int, [fp, C.c_size_t, dp, C.c_size_t, C.c_double, dp, C.c_size_t,
+                                                  C.c_size_t, C.POINTER(OConfig), C.POINTER(OCand), C.c_size_t,
+                                                  C.POINTER(C.c_uint32), fp, C.c_int, C.c_int]),
+            "oracle_median64": (C.c_double, [dp, C.c_size_t]),
+            "oracle_spectrum_f64": (C.c_size_t, [fp, C.c_size_t, C.c_double, C.POINTER(OConfig), dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -209,8 +214,30 @@ def refine(series, tsamp, frequency, span_hz, steps, bins):
     return dict(period_s=p, snr=snr.value, reduced_chi_square=chi.value, bins=vals, counts=cnts)
 
 
-def search_trials(data, nsamples, freqs, tsamp, dms, length, threads=1, **cfg):
-    """The per-trial loop over f32 channel-major data -> list of candidate lists (trial order)."""
+def median64(v):
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().oracle_median64(_d(a), len(a))
+
+
+def spectrum_f64(series, tsamp, **cfg):
+    """oracle_spectrum_f64: (raw, white) float64 arrays of next_pow2(n) + 1 bins."""
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    size = 1
+    while size < len(x):
+        size <<= 1
+    raw = np.zeros(size + 1, dtype=np.float64)
+    white = np.zeros(size + 1, dtype=np.float64)
+    c, keep = _cfg(**cfg)
+    lib().oracle_spectrum_f64(_f(x), len(x), tsamp, C.byref(c), _d(raw), _d(white))
+    del keep
+    return raw, white
+
+
+def search_trials(data, nsamples, freqs, tsamp, dms, length, threads=1, series=None, f64=False, **cfg):
+    """The per-trial loop over f32 channel-major data -> list of candidate lists (trial order).
+
+    `series`, when given, is a float32 array [len(dms), length] that receives
+    each trial's dedispersed series; f64 selects the all-f64 periodicity."""
     d = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
     f = np.ascontiguousarray(freqs, dtype=np.float64)
     dm = np.ascontiguousarray(dms, dtype=np.float64)
@@ -218,8 +245,13 @@ def search_trials(data, nsamples, freqs, tsamp, dms, length, threads=1, **cfg):
     cap = max(1, int(c.max_candidates))
     out = (OCand * (cap * len(dm)))()
     counts = np.zeros(len(dm), dtype=np.uint32)
-    rc = lib().oracle_search_trials(_f(d), nsamples, _d(f), len(f), tsamp, _d(dm), len(dm), length, C.byref(c),
-                                    out, cap, counts.ctypes.data_as(C.POINTER(C.c_uint32)), threads)
+    sp = None
+    if series is not None:
+        assert series.dtype == np.float32 and series.flags.c_contiguous and series.shape == (len(dm), length)
+        sp = _f(series)
+    rc = lib().oracle_search_trials_ex(_f(d), nsamples, _d(f), len(f), tsamp, _d(dm), len(dm), length,
+                                       C.byref(c), out, cap, counts.ctypes.data_as(C.POINTER(C.c_uint32)), sp,
+                                       1 if f64 else 0, threads)
     if rc:
         raise ValueError("invalid dedispersion input")
     del keep

@@ -139,6 +139,8 @@ EXPORTS = [
     ("psr_obs_mask", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_float, C.POINTER(Mask),
                                C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("psr_dedisperse", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
+    ("psr_apply", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.c_uint32, C.c_uint32, C.c_size_t,
+                            C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
     ("psr_search", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t, C.POINTER(C.c_uint32)]),
     ("psr_last_stage_times", C.c_int, [C.c_void_p, C.POINTER(StageTimes)]),
     ("psr_periodicity", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -146,6 +148,8 @@ EXPORTS = [
     ("psr_fold", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     ("psr_refine", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     ("psr_rules_default", None, [C.POINTER(Rules)]),
+    ("psr_measure", C.c_int, [C.POINTER(C.c_float), C.c_size_t, C.c_uint64, C.c_double, C.c_double,
+                              C.POINTER(Window), C.POINTER(Measurement)]),
     ("psr_group", C.c_int, [C.POINTER(Candidate), C.c_size_t, C.c_double, C.c_size_t, C.POINTER(Candidate),
                             C.POINTER(C.c_size_t), C.POINTER(Member), C.POINTER(C.c_size_t)]),
     ("psr_analyze", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.POINTER(Candidate),

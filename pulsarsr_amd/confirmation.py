@@ -43,6 +43,18 @@ def _cand(c):
                 score=c.score)
 
 
+def measure(x, offset, tsamp, period, window):
+    """confirmation::measure (confirmation.rs:107-141) -> Measurement dict.
+
+    `window` is a dict with start_bin, width_bins and bins."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    w = _ffi.Window(int(window["start_bin"]), int(window["width_bins"]), int(window["bins"]))
+    m = _ffi.Measurement()
+    check(_ffi.lib().psr_measure(a.ctypes.data_as(C.POINTER(C.c_float)), a.size, int(offset), float(tsamp),
+                                 float(period), C.byref(w), C.byref(m)))
+    return {k: (int(getattr(m, k)) if k.endswith("samples") else getattr(m, k)) for k, _ in _ffi.Measurement._fields_}
+
+
 def group(candidates, resolution, limit):
     """confirmation::group (confirmation.rs:177-199).
 

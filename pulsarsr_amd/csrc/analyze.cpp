@@ -136,6 +136,15 @@ void psr_rules_default(psr_rules* r) {
     *r = psr_rules{6.0, 1.0, 3, 3, 1.25, 1.5, 1.0};  // confirmation.rs:28-40
 }
 
+int psr_measure(const float* x, size_t n, uint64_t offset, double tsamp, double period,
+                const psr_window* window, psr_measurement* out) {
+    clear_error();
+    if ((!x && n) || !window || !out) return fail(PSR_E_INVALID, "null pointer");
+    if (window->bins == 0) return fail(PSR_E_CONTRACT, "measure requires a window with bins > 0");
+    measure(x, n, (size_t)offset, tsamp, period, *window, out);
+    return PSR_OK;
+}
+
 int psr_group(const psr_candidate* in, size_t n, double resolution_hz, size_t limit,
               psr_candidate* leaders, size_t* nfamilies, psr_member* members, size_t* nmembers) {
     clear_error();

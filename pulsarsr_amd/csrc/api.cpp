@@ -58,7 +58,7 @@ namespace {
 // apply()'s length rules (crates/dedisperse/src/lib.rs:72-78) for the largest DM
 // of the batch; delays are non-decreasing in DM so it bounds every trial.
 int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t length,
-                   size_t* n_out, uint64_t* maxdelay_out) {
+                   size_t* n_out, uint64_t* maxdelay_out, bool fft_limit = true) {
     double dmax = 0.0;
     for (size_t i = 0; i < ntrials; i++) {
         if (!std::isfinite(dms[i]) || dms[i] < 0.0)
@@ -74,7 +74,10 @@ int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t l
     const uint64_t available = o->nsamples - maxd;
     const uint64_t n = length ? length : available;
     if (n < 2 || n > available) return fail(PSR_E_SHAPE, "invalid dedispersion input: invalid output length");
-    if (n > (1u << 23)) return fail(PSR_E_SHAPE, "series longer than 2^23 samples is not supported");
+    if (fft_limit && n > (1u << 23))
+        return fail(PSR_E_SHAPE, "series longer than 2^23 samples is not supported");
+    if (n > (uint64_t)INT32_MAX || maxd > (uint64_t)INT32_MAX)
+        return fail(PSR_E_SHAPE, "series or delays past 2^31 samples are not supported");
     *n_out = (size_t)n;
     *maxdelay_out = maxd;
     return PSR_OK;
@@ -212,6 +215,8 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
                  hipEvent_t* ev) {
     const size_t es = p.fp16 ? 2 : 4;
     const uint32_t npart = g.m1 == 1 ? 1u : g.m2 / 2 + 1;
+    if (!fft_shape_supported(g))
+        return fail(PSR_E_SHAPE, "no FFT kernels for a " + std::to_string(g.nfft) + "-point transform");
     int rc;
     if ((rc = upload_twiddles(c, g))) return rc;
     const size_t work_bytes = g.m1 == 1 ? 8 : (size_t)fft_work_trials(g, nt) * g.m * sizeof(float2);
@@ -647,13 +652,72 @@ int psr_dedisperse(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, si
     }
     size_t n;
     uint64_t maxd;
-    if ((rc = resolve_length(o, dms, ntrials, length, &n, &maxd))) return rc;
+    if ((rc = resolve_length(o, dms, ntrials, length, &n, &maxd, false))) return rc;
     *out_len = n;
     if (!out) return PSR_OK;
-    return dedisperse_to_host(c, o, dms, ntrials, n, out);
+    if (n <= ((size_t)1 << 23)) return dedisperse_to_host(c, o, dms, ntrials, n, out);
+    // longer series than a search takes: one trial at a time in output segments
+    std::vector<uint64_t> d(o->nchans);
+    for (size_t i = 0; i < ntrials; i++) {
+        if ((rc = dedisperse::delays(o->freqs.data(), o->nchans, o->tsamp, dms[i], d.data()))) return rc;
+        if ((rc = apply_table_to_host(c, o, 0, o->nchans, d.data(), n, out + i * n))) return rc;
+    }
+    return PSR_OK;
+}
+
+int psr_apply(psr_ctx* c, psr_obs* o, const uint64_t* delays, uint32_t ch_lo, uint32_t ch_hi,
+              size_t length, float* out, size_t* out_len) {
+    clear_error();
+    if (!c || !o || !out_len || (!delays && ch_hi > ch_lo)) return fail(PSR_E_INVALID, "null pointer");
+    // apply()'s checks in its order (crates/dedisperse/src/lib.rs:68-78)
+    if (ch_hi <= ch_lo || ch_hi > o->nchans || o->nsamples == 0)
+        return fail(PSR_E_SHAPE, "invalid dedispersion input: array shape mismatch");
+    const uint32_t nch = ch_hi - ch_lo;
+    const uint64_t maxd = *std::max_element(delays, delays + nch);
+    if (maxd > o->nsamples) return fail(PSR_E_INVALID, "invalid dedispersion input: delay exceeds observation");
+    const uint64_t available = o->nsamples - maxd;
+    const uint64_t n = length ? length : available;
+    if (n < 2 || n > available) return fail(PSR_E_SHAPE, "invalid dedispersion input: invalid output length");
+    if (n > (uint64_t)INT32_MAX || maxd > (uint64_t)INT32_MAX)
+        return fail(PSR_E_SHAPE, "series or delays past 2^31 samples are not supported");
+    *out_len = (size_t)n;
+    if (!out) return PSR_OK;
+    int rc;
+    if ((rc = check_device(c))) return rc;
+    return apply_table_to_host(c, o, ch_lo, nch, delays, (size_t)n, out);
 }
 
 }  // extern "C"
+
+// out[t] = sum over c < nch, in channel order, of row (ch_lo + c)[t + delays[c]]
+// for t < n (f32 adds, as dedisperse::apply), in output segments of 2^24.
+// The caller has checked the delays and n against the observation.
+int psr::apply_table_to_host(psr_ctx* c, psr_obs* o, uint32_t ch_lo, uint32_t nch,
+                             const uint64_t* delays, size_t n, float* out) {
+    constexpr size_t SEG = (size_t)1 << 24;
+    const size_t seg = std::min(n, SEG);
+    const size_t es = sample_bytes(o);
+    int rc;
+    if ((rc = ensure(c, c->delays, (size_t)nch * sizeof(int32_t))) ||
+        (rc = ensure(c, c->series, seg * sizeof(float) + 64)) ||
+        (rc = ensure(c, c->dd_partial, ((seg + 255) / 256) * sizeof(double))))
+        return rc;
+    std::vector<int32_t> d32(nch);
+    for (uint32_t i = 0; i < nch; i++) d32[i] = (int32_t)delays[i];
+    HIP_TRY(hipMemcpyAsync(c->delays.p, d32.data(), nch * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream));
+    for (size_t t0 = 0; t0 < n; t0 += seg) {
+        const uint32_t len = (uint32_t)std::min(seg, n - t0);
+        const char* base = (const char*)o->data + ((size_t)ch_lo * o->pitch + t0) * es;
+        launch_dedisp_f32(c->stream, base, o->kind, o->pitch, nch, (const int32_t*)c->delays.p, 1, len,
+                          c->series.p, len, 0, 0.f, 1.f, (double*)c->dd_partial.p);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out + t0, c->series.p, len * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return PSR_OK;
+}
 
 int psr::dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t n,
                             float* out) {
@@ -664,7 +728,8 @@ int psr::dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t nt
     std::vector<double> sorted(ntrials);
     for (size_t i = 0; i < ntrials; i++) sorted[i] = dms[order[i]];
     const SpecGeom g = make_geometry((uint32_t)n);
-    const size_t chunk = 1024;
+    // trials per launch: up to 1024, with at most 4 GiB of series
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(1024, ((size_t)4 << 30) / ((size_t)g.series_pitch * 4)));
     std::vector<float> rowbuf;
     const Slot sl = slot_of(c, 0);
     for (size_t b = 0; b < ntrials; b += chunk) {

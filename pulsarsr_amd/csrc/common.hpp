@@ -118,6 +118,9 @@ void fill_costabs(float* host_table);
 // batches it gives the shortest headline step.
 constexpr uint64_t FFT_BATCH_BYTES = 192ull << 20;
 uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials);
+// true when launch_fft has kernels for the geometry's split (m2 of 512, 1024 or
+// 2048 in the four-step form; every power of two up to 4096 in one pass)
+bool fft_shape_supported(const SpecGeom& g);
 // whitening scales + natural-order raw spectrum; `sorted` (spectrum-sized) and
 // `floors` (one float per trial) are scratch, unused when whiten_fused(g)
 bool whiten_fused(const SpecGeom& g);

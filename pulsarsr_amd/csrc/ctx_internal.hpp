@@ -69,6 +69,10 @@ int check_device(psr_ctx* c);
 // rows out[i * n ..]; the caller has validated n against the delays.
 int dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t n,
                        float* out);
+// dedisperse::apply over channels [ch_lo, ch_lo + nch) with a caller delay
+// table (one f32 trial in channel order) into host out[n]
+int apply_table_to_host(psr_ctx* c, psr_obs* o, uint32_t ch_lo, uint32_t nch,
+                        const uint64_t* delays, size_t n, float* out);
 // recompute the FP16 storage offset/scale (m0, s0) from the device samples
 int obs_storage_scale(psr_ctx* c, psr_obs* o);
 // bytes per sample of an observation's channel rows

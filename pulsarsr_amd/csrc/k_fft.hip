@@ -834,10 +834,17 @@ static void launch_fft_t(hipStream_t s, const SpecGeom& g, const void* series, c
         const void* ser = (const char*)series + (size_t)b * g.series_pitch * es;
         void* sp = (char*)spec + (size_t)b * g.spec_pitch * es;
         double* part = partial + (size_t)b * npart;
-        if (g.m2 == 512)
-            launch_pass1<512, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
-        else
-            launch_pass1<1024, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+        switch (g.m2) {  // fft_shape_supported() admits exactly these
+            case 512:
+                launch_pass1<512, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+                break;
+            case 1024:
+                launch_pass1<1024, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+                break;
+            case 2048:
+                launch_pass1<2048, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+                break;
+        }
         switch (g.m1) {
 #define PASS2(L) \
     case L: launch_pass2<L, F16>(s, g, nb, work, sp, part, twrow, cos_tables); break;
@@ -846,6 +853,13 @@ static void launch_fft_t(hipStream_t s, const SpecGeom& g, const void* series, c
 #undef PASS2
         }
     }
+}
+
+bool fft_shape_supported(const SpecGeom& g) {
+    if (g.m1 == 1) return g.m >= 2 && g.m <= 4096 && (g.m & (g.m - 1)) == 0;
+    const bool m2_ok = g.m2 == 512 || g.m2 == 1024 || g.m2 == 2048;
+    const bool m1_ok = g.m1 >= 8 && g.m1 <= 4096 && (g.m1 & (g.m1 - 1)) == 0;
+    return m2_ok && m1_ok && (uint64_t)g.m1 * g.m2 == g.m;
 }
 
 uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials) {

@@ -127,6 +127,26 @@ def apply_dms(obs, dms, length=None):
     return out
 
 
+def apply_delays(obs, delays, length=None, channels=None):
+    """dedisperse::apply(data, nsamples, delays, length) (lib.rs:62-90), one to one.
+
+    `data` is the device observation restricted to the channel range
+    `channels` = (lo, hi) (default: all channels); `delays` has one entry per
+    channel of that range.  Returns the float32 channel-ordered sums."""
+    lo, hi = channels if channels is not None else (0, obs.nchans)
+    d = np.ascontiguousarray(delays, dtype=np.uint64)
+    if len(d) != max(hi - lo, 0):
+        raise PsrError(_ffi.PSR_E_SHAPE, "invalid dedispersion input: array shape mismatch")
+    dp = d.ctypes.data_as(C.POINTER(C.c_uint64))
+    n = C.c_size_t()
+    check(_ffi.lib().psr_apply(obs.ctx.handle, obs.handle, dp, int(lo), int(hi), int(length or 0), None,
+                               C.byref(n)))
+    out = np.zeros(n.value, dtype=np.float32)
+    check(_ffi.lib().psr_apply(obs.ctx.handle, obs.handle, dp, int(lo), int(hi), int(length or 0),
+                               out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)))
+    return out
+
+
 def apply(obs, dm, length=None):
     """dedisperse::apply(delays(freqs, tsamp, dm)) for one DM (lib.rs:62-90)."""
     return apply_dms(obs, [dm], length)[0]

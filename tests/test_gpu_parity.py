@@ -14,6 +14,7 @@ import pytest
 import oracle
 from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, fold, search, synth
 from pulsarsr_amd.core import PsrError
+from matching import match_fp16_vs_f64, match_lists
 
 pytestmark = pytest.mark.gpu
 
@@ -109,6 +110,69 @@ def test_dedisperse_full_length_and_unsorted_dms(ctx):
         assert (got[i] == ref).all()
 
 
+def test_apply_delay_table_matches_oracle(ctx):
+    """psr_apply, 1:1 with dedisperse::apply(data, nsamples, delays, length): the
+    full band with a DM's delays, subband channel slices with the full band's
+    delays (confirmation.rs:362-372), all-zero tables (confirmation.rs:253,321),
+    arbitrary non-monotonic delays, u8 and float observations -- bit-exact."""
+    rng = np.random.default_rng(62)
+    nch, ns = 48, 20000
+    payload = rng.integers(0, 256, (ns, nch), dtype=np.uint8).tobytes()
+    obs = dedisperse.Observation.from_packed(payload, 8, nch, 64e-6, 1500.0, -2.0)
+    data, _ = oracle.decode(payload, 8, nch)
+    freqs = 1500.0 - 2.0 * np.arange(nch)
+    d = oracle.delays(freqs, 64e-6, 55.0)
+    x = dedisperse.apply_delays(obs, d)
+    assert (x == oracle.apply(data, ns, d)).all()
+    n = 9000
+    for lo, hi in ((0, 12), (12, 24), (24, 36), (36, 48), (5, 6)):
+        sub = data[lo * ns:hi * ns]
+        got = dedisperse.apply_delays(obs, d[lo:hi], n, channels=(lo, hi))
+        assert (got.view(np.uint32) == oracle.apply(sub, ns, d[lo:hi], n).view(np.uint32)).all(), (lo, hi)
+    z = np.zeros(nch, np.uint64)
+    assert (dedisperse.apply_delays(obs, z, n) == oracle.apply(data, ns, z, n)).all()
+    arb = rng.integers(0, 3000, nch).astype(np.uint64)
+    assert (dedisperse.apply_delays(obs, arb) == oracle.apply(data, ns, arb)).all()
+    f = (rng.standard_normal(nch * 5000) * 3.0 + 7.0).astype(np.float32)
+    fobs = dedisperse.Observation.from_f32(f, 5000, 64e-6, 1500.0, -2.0)
+    arb = rng.integers(0, 700, nch).astype(np.uint64)
+    got = dedisperse.apply_delays(fobs, arb[10:30], channels=(10, 30))
+    ref = oracle.apply(f[10 * 5000:30 * 5000], 5000, arb[10:30])
+    assert (got.view(np.uint32) == ref.view(np.uint32)).all()
+    # apply()'s errors (crates/dedisperse/src/lib.rs:67-78)
+    with pytest.raises(PsrError, match="array shape mismatch"):
+        dedisperse.apply_delays(obs, d[:3], channels=(0, 4))
+    with pytest.raises(PsrError, match="array shape mismatch"):
+        dedisperse.apply_delays(obs, [], channels=(4, 4))
+    with pytest.raises(PsrError, match="delay exceeds observation"):
+        dedisperse.apply_delays(obs, np.full(nch, ns + 1, np.uint64))
+    with pytest.raises(PsrError, match="invalid output length"):
+        dedisperse.apply_delays(obs, d, ns)
+    with pytest.raises(PsrError, match="invalid output length"):
+        dedisperse.apply_delays(obs, d, 1)
+    obs.close()
+    fobs.close()
+
+
+def test_dedisperse_longer_than_a_search_series(ctx):
+    """apply() has no length limit: a series past 2^23 samples (longer than any
+    search takes) is dedispersed in output segments, bit-exact."""
+    rng = np.random.default_rng(23)
+    nch, ns = 6, (1 << 23) + 70000
+    payload = rng.integers(0, 16, (ns, nch), dtype=np.uint8)
+    packed = (payload[:, 0::2] | (payload[:, 1::2] << 4)).astype(np.uint8).tobytes()
+    obs = dedisperse.Observation.from_packed(packed, 4, nch, 64e-6, 1500.0, -20.0)
+    data, ns2 = oracle.decode(packed, 4, nch)
+    assert ns2 == ns
+    freqs = 1500.0 - 20.0 * np.arange(nch)
+    for dm in (0.0, 300.0):
+        got = dedisperse.apply(obs, dm)
+        ref = oracle.apply(data, ns, oracle.delays(freqs, 64e-6, dm))
+        assert len(got) == len(ref) > (1 << 23)
+        assert (got == ref).all(), dm
+    obs.close()
+
+
 def test_dispersed_delta_collapses(ctx):
     # crates/dedisperse/src/lib.rs:95-108 through the GPU
     for freqs in ([1400.0, 1300.0, 1200.0], [1200.0, 1300.0, 1400.0]):
@@ -135,7 +199,8 @@ def test_f32_channel_order_sums(ctx):
         assert (got[i].view(np.uint32) == ref.view(np.uint32)).all()
 
 
-@pytest.mark.parametrize("n", [1000, 4096, 8000, 32583, 100000, 263500, 1500000, 2097152, 2300000])
+@pytest.mark.parametrize("n", [1000, 4096, 8000, 32583, 100000, 263500, 1500000, 2097152, 2300000, 4194305,
+                               8388608])
 def test_spectrum_matches_oracle(ctx, n):
     rng = np.random.default_rng(n)
     t = np.arange(n)
@@ -150,7 +215,7 @@ def test_spectrum_matches_oracle(ctx, n):
     assert np.linalg.norm(white - owhite) <= 1e-5 * np.linalg.norm(owhite)
 
 
-@pytest.mark.parametrize("n", [263500, 1500000, 2097152])
+@pytest.mark.parametrize("n", [263500, 1500000, 2097152, 4194305, 8388608])
 def test_fp16_whitening_scales_are_exact_window_medians(ctx, n):
     # FP16 storage (packed 16-bit block sort, windowed cut search): every 64-bin
     # block's scale is max(max(upper median / ln2, floor), 1e-20) of the stored
@@ -177,30 +242,6 @@ def test_fp16_whitening_scales_are_exact_window_medians(ctx, n):
     exp[0] = 0.0
     bad = np.nonzero(white != exp)[0]
     assert bad.size == 0, (bad[:8], white[bad[:8]], exp[bad[:8]])
-
-
-def match_lists(got, ref, threshold, max_candidates=32):
-    """Per-trial candidate lists agree up to threshold-edge flips; returns worst rel score error."""
-    worst = 0.0
-    for t, (g, r) in enumerate(zip(got, ref)):
-        rs = [(c["harmonics"], c["frequency_hz"], c["score"]) for c in r]
-        gs = [(c.harmonics, c.frequency_hz, c.score) for c in g]
-        used = set()
-        for h, f, s in gs:
-            hit = None
-            for j, (h2, f2, s2) in enumerate(rs):
-                if j not in used and h2 == h and abs(f2 - f) <= 1e-6 * max(1.0, f):
-                    hit = j
-                    break
-            if hit is None:
-                assert abs(s - threshold) < 1e-3 * threshold or len(gs) >= max_candidates, (t, h, f, s)
-                continue
-            used.add(hit)
-            worst = max(worst, abs(s - rs[hit][2]) / abs(rs[hit][2]))
-        for j, (h2, f2, s2) in enumerate(rs):
-            if j not in used:
-                assert abs(s2 - threshold) < 1e-3 * threshold or len(rs) >= max_candidates, (t, h2, f2, s2)
-    return worst
 
 
 @pytest.mark.parametrize("nbits", [4, 8])
@@ -282,20 +323,22 @@ def test_search_is_deterministic(ctx):
 
 
 def test_fp16_storage_error_vs_fp64_oracle(ctx):
+    """FP16 storage against the all-f64 oracle on every trial: every candidate
+    matched (or one the FP16 rounding can flip), at least 90 % matched, the
+    strongest candidate matched in at least 90 % of the trials, SNR within 1e-2."""
     w = small_workload(8, nsamples=1 << 16)
     _, obs, data, ns = setup(w)
     n = disc_len(w)
     cfg = search.Config(min_hz=w.min_hz, threshold=w.threshold, precision=PRECISION_FP16)
     got = search.search(obs, w.dms, n, cfg)
-    worst = 0.0
-    for i, dm in enumerate(w.dms):
-        x = oracle.apply(data, ns, oracle.delays(w.frequencies(), w.tsamp, dm), n)
-        ref = oracle.periodicity(x, w.tsamp, dm, f64=True, min_hz=w.min_hz, threshold=w.threshold)
-        for g in got[i][:3]:
-            hit = [r for r in ref if r["harmonics"] == g.harmonics and abs(r["frequency_hz"] - g.frequency_hz) < 1e-4 * g.frequency_hz]
-            if hit:
-                worst = max(worst, abs(g.score - hit[0]["score"]) / hit[0]["score"])
-    assert worst <= 1e-2
+    ref = oracle.search_trials(data, ns, w.frequencies(), w.tsamp, w.dms, n, threads=4, f64=True,
+                               min_hz=w.min_hz, threshold=w.threshold)
+    nfft = 2 * (1 << (n - 1).bit_length())
+    m = match_fp16_vs_f64(got, ref, w.threshold, 1.0 / (nfft * w.tsamp), 1.0 / (n * w.tsamp))
+    print("fp16 vs f64: %r" % m, flush=True)
+    assert m["total"] >= 2 * len(w.dms) and m["matched"] >= 0.9 * m["total"]
+    assert m["trials"] >= len(w.dms) // 2 and m["top_matched"] >= 0.9 * m["trials"]
+    assert m["max_rel_snr_error"] <= 1e-2
 
 
 def test_fold_bins_bit_exact(ctx):

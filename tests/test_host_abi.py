@@ -2,6 +2,7 @@
 include/psr_gpu.h, and its host-side numerics (no GPU) agree with the oracle.
 No compute call here touches a GPU."""
 import json
+import math
 import os
 import re
 import struct
@@ -131,3 +132,32 @@ def test_group_matches_oracle_and_rules_default():
         want = oracle.group(cands, 0.01, limit)
         assert got == want
     assert confirmation.default_rules() == oracle.DEFAULT_RULES
+
+
+def test_measure_matches_oracle_and_closed_form():
+    # confirmation::measure (confirmation.rs:107-141) through psr_measure
+    from pulsarsr_amd import confirmation
+    rng = np.random.default_rng(107)
+    x = (rng.standard_normal(30000) * 5.0 + 100.0).astype(np.float32)
+    tsamp, period = 64e-6, 0.0123
+    for win, off in (({"start_bin": 10, "width_bins": 4, "bins": 64}, 0),
+                     ({"start_bin": 62, "width_bins": 8, "bins": 64}, 12345),  # wraps past bin 63
+                     ({"start_bin": 0, "width_bins": 1, "bins": 32}, 7)):
+        got = confirmation.measure(x, off, tsamp, period, win)
+        assert got == oracle.measure(x, off, tsamp, period, win)
+        # closed form: Welch's two-sample statistic over the phase-binned samples
+        ph = (np.arange(len(x)) + off) * tsamp / period
+        b = np.minimum(((ph - np.trunc(ph)) * win["bins"]).astype(np.int64), win["bins"] - 1)
+        on = (b + win["bins"] - win["start_bin"]) % win["bins"] < win["width_bins"]
+        a, o = x[on].astype(np.float64), x[~on].astype(np.float64)
+        d = a.mean() - o.mean()
+        se = math.sqrt(a.var(ddof=1) / len(a) + o.var(ddof=1) / len(o))
+        assert got["on_samples"] == len(a) and got["off_samples"] == len(o)
+        assert got["mean_difference"] == pytest.approx(d, rel=1e-9, abs=1e-9)
+        assert got["standard_error"] == pytest.approx(se, rel=1e-6)
+        assert got["snr"] == pytest.approx(d / se, rel=1e-6, abs=1e-6)
+    # fewer than two on-samples: the zero measurement (confirmation.rs:118-126)
+    z = confirmation.measure(x[:3], 0, tsamp, period, {"start_bin": 0, "width_bins": 1, "bins": 64})
+    assert z["snr"] == 0.0 and z["mean_difference"] == 0.0 and z["on_samples"] + z["off_samples"] == 3
+    with pytest.raises(PsrError):
+        confirmation.measure(x, 0, tsamp, period, {"start_bin": 0, "width_bins": 1, "bins": 0})

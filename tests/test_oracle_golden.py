@@ -322,3 +322,158 @@ def test_refine_keeps_last_maximum():
     x = np.ones(1000, np.float32)
     r = oracle.refine(x, 0.001, 10.0, 1.0, 4, 8)
     assert r["period_s"] == 1.0 / (10.0 - 1.0 + 2.0 * 1.0 * 4 / 4)
+
+
+# ------------------------------------------------------- FP64 oracle pins
+def _round_half_away(p):
+    f = math.floor(p)
+    return int(f) + (1 if p - f >= 0.5 else 0)
+
+
+def numpy_periodicity_f64(x, tsamp, dm, min_hz=1.0, max_hz=math.inf, threshold=6.0, max_candidates=32,
+                          birdies=(), max_harmonics=16):
+    """crates/search/src/lib.rs:33-145 in float64, built on numpy's FFT (pocketfft)
+    and numpy's sort for the upper median -- independent of the C oracle's
+    transform and selection."""
+    n = len(x)
+    size = 2 * (1 << (n - 1).bit_length())
+    xd = x.astype(np.float64)
+    mean = float(np.sum(xd)) / n
+    inp = np.zeros(size)
+    inp[:n] = xd - mean
+    X = np.fft.rfft(inp)
+    raw = X.real * X.real + X.imag * X.imag
+    nb = len(raw)
+    floor = float(np.sum(raw)) / nb * 1e-6
+    pw = raw.copy()
+    for b in range(0, nb, 64):
+        lo, hi = max(b - 128, 1), min(b + 192, nb)
+        med = np.sort(raw[lo:hi])[(hi - lo) // 2]
+        pw[b:b + 64] /= max(max(med / math.log(2.0), floor), 1e-20)
+    pw[0] = 0.0
+    bin_hz = 1.0 / (size * tsamp)
+    f = np.arange(nb) * bin_hz
+    for c, w in birdies:
+        pw[np.abs(f - c) <= w] = 0.0
+    first = max(math.ceil(min_hz / bin_hz), 1)
+    last = min(int(math.floor(min(max_hz, 0.5 / tsamp) / bin_hz)), nb - 2)
+
+    def offset(k):
+        curv = raw[k - 1] - 2.0 * raw[k] + raw[k + 1]
+        return min(max(0.5 * (raw[k - 1] - raw[k + 1]) / curv, -0.5), 0.5) if curv < 0.0 else 0.0
+
+    cands = []
+    for h in (1, 2, 4, 8, 16):
+        if h > max_harmonics:
+            break
+        end = min(last, (nb - 2) // h)
+        if end <= first + 1:
+            continue
+        sc = np.full(end - first + 1, -math.inf)
+        for k in range(first, end + 1):
+            if h > 1 and pw[k] < 4.0:
+                continue
+            o = offset(k)
+            s = 0.0
+            for j in range(1, h + 1):
+                s += pw[min(_round_half_away((k + o) * j), nb - 1)]
+            sc[k - first] = (s - h) / math.sqrt(h)
+        for i in range(1, len(sc) - 1):
+            if sc[i] >= threshold and sc[i] > sc[i - 1] and sc[i] >= sc[i + 1]:
+                k = first + i
+                fr = (k + offset(k)) * bin_hz
+                cands.append(dict(frequency_hz=fr, period_s=1.0 / fr, dm=dm, harmonics=h, score=float(sc[i])))
+    cands.sort(key=lambda c: -c["score"])  # stable, as sort_by with total_cmp
+    kept = []
+    for c in cands:
+        if any(abs(k["frequency_hz"] - c["frequency_hz"]) < 1.0 / (n * tsamp) for k in kept):
+            continue
+        kept.append(c)
+        if len(kept) >= max_candidates:
+            break
+    return kept
+
+
+def test_median64_is_upper_median():
+    # crates/rfi/src/lib.rs:4-12 in double: v[len/2] of the sorted slice
+    rng = np.random.default_rng(64)
+    assert oracle.median64([]) == 0.0
+    assert oracle.median64([2.0, 1.0]) == 2.0 and oracle.median64([5.0, 1.0, 3.0]) == 3.0
+    for n in (1, 2, 3, 63, 64, 255, 256, 320, 1001):
+        v = rng.exponential(size=n)
+        assert oracle.median64(v) == np.sort(v)[n // 2], n
+    v = np.array([1.0, 1.0, 0.5, 1.0, 2.0, 2.0])  # ties
+    assert oracle.median64(v) == np.sort(v)[3]
+
+
+def test_spectrum_f64_matches_numpy():
+    x = (np.random.default_rng(11).standard_normal(3000) * 16 + 2000).astype(np.float32)
+    raw, white = oracle.spectrum_f64(x, 64e-6, min_hz=10.0, birdies=[(500.0, 20.0)])
+    xd = x.astype(np.float64)
+    inp = np.zeros(8192)
+    inp[:3000] = xd - xd.sum() / 3000
+    X = np.fft.rfft(inp)
+    ref = X.real ** 2 + X.imag ** 2
+    assert np.max(np.abs(raw - ref)) <= 1e-12 * ref.max()
+    f = np.arange(len(raw)) / (8192 * 64e-6)
+    assert white[0] == 0.0 and np.all(white[np.abs(f - 500.0) <= 20.0] == 0.0)
+
+
+@pytest.mark.parametrize("case", ["pulse_train", "sine_noise_birdie", "pulse_train_h8"])
+def test_periodicity_f64_matches_numpy_restatement(case):
+    if case.startswith("pulse_train"):
+        x, dt, _ = xorshift_pulse_train()
+        kw = dict(max_harmonics=8) if case.endswith("h8") else {}
+        args = (x, dt, 18.0)
+    else:
+        x, dt, f = _sine_series()
+        x = x + np.random.default_rng(6).standard_normal(len(x)).astype(np.float32) * np.float32(0.5)
+        kw = dict(min_hz=10.0, birdies=[(100.0, 2.0)], threshold=4.0)
+        args = (x, dt, 0.0)
+    got = oracle.periodicity(*args, f64=True, **kw)
+    ref = numpy_periodicity_f64(*args, **kw)
+    assert len(ref) > 3 and len(got) == len(ref)
+    for g, r in zip(got, ref):
+        assert g["harmonics"] == r["harmonics"]
+        assert abs(g["frequency_hz"] - r["frequency_hz"]) <= 1e-9 * r["frequency_hz"]
+        assert abs(g["score"] - r["score"]) <= 1e-9 * abs(r["score"])
+
+
+def test_periodicity_f64_reproduces_synthetic_report_members():
+    # the FP64 twin on synthetic.fil: every one of the 549 saved members is found
+    # (same trial, harmonics, frequency within 1e-5) with its f32-computed score
+    # within 1e-4 (the reference rounds its spectrum to f32; the twin does not)
+    path = reference_file("synthetic.fil")
+    g = golden("synthetic_report.json")
+    s = g["settings"]
+    raw = open(path, "rb").read()
+    off = golden("headers.json")["synthetic_header"]["data_offset"]
+    data, ns = oracle.decode(raw[off:], 32, 32)
+    freqs = 1450.0 - 4.0 * np.arange(32)
+    dms = oracle.plan(s["dm_min"], s["dm_max"], s["dm_step"])
+    res = oracle.search_trials(data, ns, freqs, s["tsamp"], dms, s["discovery_samples"], threads=4, f64=True,
+                               min_hz=s["min_hz"], max_hz=s["max_hz"], threshold=s["threshold"])
+    worst, checked = _match_members(res, g["members"], s["dm_step"])
+    assert checked == len(g["members"]) == 549
+    assert worst < 1e-4
+
+
+def test_periodicity_f64_reproduces_gbt_report_members():
+    # a sample of the GBT report's members (after its RFI mask), FP64 twin
+    path = reference_file("GBT_Lband_PSR.fil")
+    g = golden("gbt_report.json")
+    s = g["settings"]
+    h = golden("headers.json")["gbt_header"]
+    raw = np.fromfile(path, dtype=np.uint8)
+    data, ns = oracle.decode(raw[h["data_offset"]:], h["nbits"], h["nchans"])
+    oracle.mask(data, ns, int(max(0.1 / s["tsamp"], 32)), 8.0)
+    freqs = h["fch1"] + np.arange(h["nchans"]) * h["foff"]
+    dms = oracle.plan(s["dm_min"], s["dm_max"], s["dm_step"])
+    d = oracle.delays(freqs, s["tsamp"], s["dm_max"])
+    disc = (ns - 2 * int(d.max())) // 2
+    trials = sorted(set(m[1] for m in g["members"]))[7::40]
+    idx = [int(round(t / s["dm_step"])) for t in trials]
+    res = oracle.search_trials(data, ns, freqs, s["tsamp"], dms[idx], disc, threads=4, f64=True,
+                               min_hz=s["min_hz"], max_hz=s["max_hz"], threshold=s["threshold"])
+    worst, checked = _match_members(res, g["members"], s["dm_step"])
+    assert checked > 100 and worst < 1e-4
