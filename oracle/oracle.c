@@ -161,6 +161,29 @@ int oracle_apply(const float *data, size_t nsamples, size_t nch, const uint64_t 
 }
 
 /* crates/fbank/src/lib.rs:149-196 (payload decode, time-major -> channel-major) */
+typedef struct {
+    const uint8_t *payload;
+    size_t nchans, ns;
+    float *out;
+} decode8_job;
+typedef struct {
+    decode8_job *j;
+    size_t t0, t1;
+} decode8_part;
+static void *decode8_worker(void *arg) {
+    decode8_part *p = (decode8_part *)arg;
+    const decode8_job *j = p->j;
+    for (size_t t0 = p->t0; t0 < p->t1; t0 += 256) {
+        size_t t1 = t0 + 256 < p->t1 ? t0 + 256 : p->t1;
+        for (size_t ch = 0; ch < j->nchans; ch++) {
+            float *o = j->out + ch * j->ns;
+            const uint8_t *q = j->payload + ch;
+            for (size_t t = t0; t < t1; t++) o[t] = (float)q[t * j->nchans];
+        }
+    }
+    return NULL;
+}
+
 int oracle_decode(const uint8_t *payload, size_t payload_len, int nbits, size_t nchans,
                   size_t limit, float *out, size_t *nsamples_out) {
     if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8 || nbits == 16 || nbits == 32))
@@ -171,8 +194,25 @@ int oracle_decode(const uint8_t *payload, size_t payload_len, int nbits, size_t 
     size_t total = bits / bits_per_frame;
     size_t ns = limit ? (limit < total ? limit : total) : total;
     if (ns < 2) return -1;
-    for (size_t t = 0; t < ns; t++)
-        for (size_t ch = 0; ch < nchans; ch++) {
+    /* frames in tiles of 256 so the channel-major stores run contiguously */
+    if (nbits == 8) { /* every byte is a sample: no checks needed; split over threads */
+        decode8_job j = {payload, nchans, ns, out};
+        size_t nth = ns * nchans > ((size_t)1 << 26) ? 16 : 1;
+        pthread_t th[16];
+        decode8_part parts[16];
+        for (size_t i = 0; i < nth; i++) {
+            parts[i].j = &j;
+            parts[i].t0 = (ns * i / nth) & ~(size_t)255;
+            parts[i].t1 = i + 1 == nth ? ns : (ns * (i + 1) / nth) & ~(size_t)255;
+            pthread_create(&th[i], NULL, decode8_worker, &parts[i]);
+        }
+        for (size_t i = 0; i < nth; i++) pthread_join(th[i], NULL);
+        *nsamples_out = ns;
+        return 0;
+    }
+    for (size_t t0 = 0; t0 < ns; t0 += 256)
+        for (size_t ch = 0; ch < nchans; ch++)
+            for (size_t t = t0; t < ns && t < t0 + 256; t++) {
             size_t i = t * nchans + ch;
             float v;
             if (nbits <= 4) {

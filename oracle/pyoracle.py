@@ -117,7 +117,7 @@ def decode(payload, nbits, nchans, limit=0):
     p = np.ascontiguousarray(np.frombuffer(bytes(payload), dtype=np.uint8) if not isinstance(payload, np.ndarray) else payload)
     total = p.nbytes * 8 // (nchans * nbits) if nchans * nbits else 0
     ns_cap = min(limit, total) if limit else total
-    out = np.zeros(max(1, ns_cap * nchans), dtype=np.float32)
+    out = np.empty(max(1, ns_cap * nchans), dtype=np.float32)
     ns = C.c_size_t()
     if lib().oracle_decode(p.ctypes.data, p.nbytes, nbits, nchans, int(limit), _f(out), C.byref(ns)):
         raise ValueError("invalid or unsupported filterbank")

@@ -4,7 +4,8 @@ dedisperse, header.  Run as `python -m pulsarsr_amd <command> ...`.
 Arguments, defaults, validation messages, console lines and the JSON outputs
 (header.json, report.json schema 2, fold.json, series.json/series.f32) follow
 the reference; the PNG plots and the `inspect`/`synth` commands are not
-provided.  Every stage runs on the GPU through libpsrgpu: packed ingest,
+provided.  `search` adds three flags the reference lacks (its behaviour is the
+default of each): --max-harmonics, --precision and --dm-plan.  Every stage runs on the GPU through libpsrgpu: packed ingest,
 rfi::mask, the DM-trial search, confirmation::analyze, dedispersion and folds.
 """
 import argparse
@@ -18,7 +19,7 @@ import time
 
 import numpy as np
 
-from . import PRECISION_FP32, confirmation, dedisperse, fbank, fold, report, rfi, search
+from . import PRECISION_FP16, PRECISION_FP32, confirmation, dedisperse, fbank, fold, report, rfi, search
 from .core import PsrError
 
 VERSION = "0.1.0"
@@ -111,8 +112,12 @@ def cmd_search(a):
     tsamp = hdr.tsamp
     report.write(os.path.join(a.out, "header.json"), header_json(a.file, hdr, raw))
     mask = None if a.no_rfi else rfi.mask(obs, max(int(0.1 / tsamp), 32), 8.0)
-    step = a.dm_step if a.dm_step is not None else dedisperse.dm_step(freqs, tsamp)
-    trials = dedisperse.plan(a.dm_min, a.dm_max, step)
+    if a.dm_plan is not None:
+        trials, step = a.dm_plan, a.dm_plan_step
+        a.dm_min, a.dm_max = float(trials[0]), float(trials[-1])
+    else:
+        step = a.dm_step if a.dm_step is not None else dedisperse.dm_step(freqs, tsamp)
+        trials = dedisperse.plan(a.dm_min, a.dm_max, step)
     max_delay = int(dedisperse.delays(freqs, tsamp, a.dm_max).max())
     if obs.nsamples < max_delay or obs.nsamples - max_delay < 512:
         raise SystemExit("Error: \"DM range leaves fewer than 512 samples\"")
@@ -121,8 +126,10 @@ def cmd_search(a):
         raise SystemExit("Error: \"DM range leaves no independent discovery/confirmation split\"")
     discovery = (length - max_delay) // 2
     cfg = search.Config(min_hz=a.min_hz, max_hz=a.max_hz, threshold=a.threshold, max_candidates=32,
-                        birdies=list(a.birdie), precision=PRECISION_FP32)
-    threads = a.threads or 1
+                        birdies=list(a.birdie), max_harmonics=a.max_harmonics,
+                        precision=PRECISION_FP16 if a.precision == "fp16" else PRECISION_FP32)
+    # rayon's default pool: one worker per available core (crates/cli/src/main.rs:407-414)
+    threads = a.threads or len(os.sched_getaffinity(0))
     print("%d samples, %d channels, %d DM trials (step %.6f), %d threads"
           % (obs.nsamples, hdr.nchans, len(trials), step, threads))
     lists = search.search(obs, trials, discovery, cfg)
@@ -212,6 +219,28 @@ def _birdie(s):
     return (f, w)
 
 
+def _dm_plan(s):
+    """--dm-plan LO:HI:STEP[,LO:HI:STEP...]: dedisperse::plan over each segment,
+    concatenated (a step-doubling plan); the trials must increase."""
+    out, steps = [], []
+    for seg in s.split(","):
+        parts = seg.split(":")
+        if len(parts) != 3:
+            raise argparse.ArgumentTypeError("expected LO:HI:STEP segments separated by commas")
+        try:
+            lo, hi, st = (float(v) for v in parts)
+        except ValueError:
+            raise argparse.ArgumentTypeError("invalid DM plan segment %r" % seg)
+        try:
+            out.extend(dedisperse.plan(lo, hi, st).tolist())
+        except PsrError as e:
+            raise argparse.ArgumentTypeError(str(e))
+        steps.append(st)
+    if any(b <= a for a, b in zip(out, out[1:])):
+        raise argparse.ArgumentTypeError("DM plan segments must increase")
+    return np.array(out, dtype=np.float64), steps[0]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="psr", description="GPU filterbank pulsar search (PulsarSr drop-in)")
     sub = ap.add_subparsers(dest="command", required=True)
@@ -242,7 +271,16 @@ def main(argv=None):
     s.add_argument("--max-samples", type=int, default=None)
     s.add_argument("--threads", type=int, default=None)
     s.add_argument("--out", default="output")
+    # knobs the reference hard-codes (16 harmonics, FP32, one uniform plan)
+    s.add_argument("--max-harmonics", type=int, choices=[1, 2, 4, 8, 16], default=16,
+                   help="highest harmonic sum (the reference always sums 1, 2, 4, 8 and 16)")
+    s.add_argument("--precision", choices=["fp32", "fp16"], default="fp32",
+                   help="storage of dedispersed series and spectra (fp16: per-trial scaled half precision)")
+    s.add_argument("--dm-plan", type=_dm_plan, default=None, metavar="LO:HI:STEP[,...]",
+                   help="segmented DM plan (e.g. step doubling); overrides --dm-min/--dm-max/--dm-step")
     a = ap.parse_args(argv)
+    if a.command == "search":
+        a.dm_plan, a.dm_plan_step = a.dm_plan if a.dm_plan is not None else (None, None)
     try:
         return {"search": cmd_search, "fold": cmd_fold, "dedisperse": cmd_dedisperse, "header": cmd_header}[a.command](a)
     except PsrError as e:

@@ -89,6 +89,11 @@ def _plan(lo, hi, step):
     return lo + np.arange(n, dtype=np.float64) * step
 
 
+# the precision workload's PRESTO-style plan: 1500 + 750 + 750 trials, the step
+# doubling at 150 and 300 pc cm^-3
+PRECISION_DM_PLAN = ((0.0, 149.95, 0.1), (150.0, 299.9, 0.2), (300.0, 599.8, 0.4))
+
+
 def workload(name, nsamples=None):
     """The named benchmark workloads; `nsamples` overrides the length (tests)."""
     if name == "plain-64ch-8bit":  # CPU-runnable reference case
@@ -127,7 +132,8 @@ def workload(name, nsamples=None):
             red = 0.0
         else:
             rfi, bursts, red = [], [], 0.2
-            dms = np.concatenate([_plan(0.0, 149.9, 0.1), 150.0 + np.arange(750) * 0.2, 300.0 + np.arange(750) * 0.4])
+            # step doubling (psr search --dm-plan 0:149.95:0.1,150:299.9:0.2,300:599.8:0.4)
+            dms = np.concatenate([_plan(*seg) for seg in PRECISION_DM_PLAN])
         return Workload(name, 11 if headline else 10, n, 1024, 8, 64e-6, 1599.8046875, -0.390625, 128.0, 16.0,
                         pulsars=pulsars, bursts=bursts, rfi_chans=rfi, rfi_amp=3.0 if rfi else 0.0,
                         red_noise=red, dms=dms, max_harmonics=16, top=20 if headline else 5)
