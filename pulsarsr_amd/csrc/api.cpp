@@ -126,9 +126,11 @@ int upload_twiddles(psr_ctx* c, const SpecGeom& g) {
     return PSR_OK;
 }
 
+}  // namespace
+
 // search::periodicity's bin limits (crates/search/src/lib.rs:73-80) and birdie
 // bins (lib.rs:62-72) for one geometry.
-int make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, SearchParams* p,
+int psr::make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, SearchParams* p,
                 std::vector<uint32_t>* birdie_bins) {
     std::memset(p, 0, sizeof(*p));
     p->tsamp = tsamp;
@@ -192,6 +194,8 @@ int make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, Sear
     return PSR_OK;
 }
 
+namespace {
+
 // A trial-series slot: dedispersed series, their means/corrections and DMs.
 struct Slot {
     DevBuf* series;
@@ -204,11 +208,51 @@ Slot slot_of(psr_ctx* c, int i) {
                   : Slot{&c->series_b, &c->mean32_b, &c->corr16_b, &c->dms_b};
 }
 
+}  // namespace
+
 // Enqueue the spectral chain (FFT, whitening, harmonic sums, ranking) over `nt`
-// series of slot `sl` on stream `s`, ending with the copy of the ranked
-// candidates to host `out` and of the counts (nt + 1 words, the last one the
-// peak-overflow count) to host `counts`.  Nothing waits: the caller synchronises.
-// With `ev`, events ev[0..4] bracket FFT, whitening, harmonic sums and ranking.
+// series in b.series on stream `s`; the ranked candidates land in b.cands and
+// the counts (nt + 1 words, the last one the peak-overflow count) in b.counts.
+// Allocates nothing, copies nothing to the host and waits for nothing.  With
+// `ev`, events ev[0..3] bracket FFT, whitening, harmonic sums and ranking.
+int psr::enqueue_spectral(const SpecGeom& g, const SearchParams& p,
+                          const std::vector<uint32_t>& birdies, const SpecBufs& b, uint32_t nt,
+                          size_t cap_per_trial, hipStream_t s, hipEvent_t* ev) {
+    const uint32_t npart = g.m1 == 1 ? 1u : g.m2 / 2 + 1;
+    std::vector<uint32_t> bh(2 * MAX_BIRDIE_INTERVALS, 0);
+    std::copy(birdies.begin(), birdies.end(), bh.begin());
+    // counters cleared by a kernel (no memset nodes if a caller captures the stream)
+    launch_fill_u32(s, b.peak_counts, nt, 0u);
+    launch_fill_u32(s, b.counts, nt + 1, 0u);
+    if (ev) HIP_TRY(hipEventRecord(ev[0], s));
+    launch_fft(s, g, b.series, (int)p.fp16, b.mean32, b.corr16, nt, b.work, b.spec_t, b.fft_partial,
+               b.twiddles, b.costabs);
+    HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[1], s));
+    launch_whiten(s, g, b.spec_t, (int)p.fp16, nt, b.fft_partial, npart, b.raw_nat, b.blk_sorted,
+                  b.floors, b.scales);
+    HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[2], s));
+    launch_harmsum(s, g, p, b.raw_nat, b.scales, bh.data(), nt, b.peaks, b.peak_counts);
+    HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[3], s));
+    launch_finalize(s, g, p, b.dms, nt, b.peaks, b.peak_counts, b.cands, (uint32_t)cap_per_trial,
+                    b.counts);
+    HIP_TRY(hipGetLastError());
+    return PSR_OK;
+}
+
+size_t psr::spectral_work_bytes(const SpecGeom& g, uint32_t nt) {
+    return g.m1 == 1 ? 8 : (size_t)fft_work_trials(g, nt) * g.m * sizeof(float2);
+}
+
+namespace {
+
+// Enqueue the spectral chain over `nt` series of slot `sl` on stream `s` into
+// the ctx's buffers, ending with the copy of the ranked candidates to host `out`
+// and of the counts (nt + 1 words) to host `counts`.  Nothing waits: the caller
+// synchronises.  With `ev`, events ev[0..4] bracket FFT, whitening, harmonic
+// sums and ranking, ev[4] after the copies.
 int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
                  const std::vector<uint32_t>& birdies, const Slot& sl, uint32_t nt,
                  psr_candidate* out, size_t cap_per_trial, uint32_t* counts, hipStream_t s,
@@ -219,8 +263,7 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
         return fail(PSR_E_SHAPE, "no FFT kernels for a " + std::to_string(g.nfft) + "-point transform");
     int rc;
     if ((rc = upload_twiddles(c, g))) return rc;
-    const size_t work_bytes = g.m1 == 1 ? 8 : (size_t)fft_work_trials(g, nt) * g.m * sizeof(float2);
-    if ((rc = ensure(c, c->work, work_bytes)) ||
+    if ((rc = ensure(c, c->work, spectral_work_bytes(g, nt))) ||
         (rc = ensure(c, c->spec_t, (size_t)nt * g.spec_pitch * es)) ||
         (rc = ensure(c, c->raw_nat, (size_t)nt * g.spec_pitch * es)) ||
         (rc = ensure(c, c->scales, (size_t)nt * g.nblocks * sizeof(float))) ||
@@ -230,31 +273,27 @@ int run_spectral(psr_ctx* c, const SpecGeom& g, const SearchParams& p,
         (rc = ensure(c, c->peaks, (size_t)nt * PEAK_CAP * sizeof(RawPeak))) ||
         (rc = ensure(c, c->peak_counts, (size_t)nt * sizeof(uint32_t))) ||
         (rc = ensure(c, c->cands, (size_t)nt * cap_per_trial * sizeof(CandidateOut))) ||
-        (rc = ensure(c, c->counts, (size_t)(nt + 1) * sizeof(uint32_t))) ||
-        (rc = ensure(c, c->birdies, std::max<size_t>(birdies.size(), 1) * sizeof(uint32_t))))
+        (rc = ensure(c, c->counts, (size_t)(nt + 1) * sizeof(uint32_t))))
         return rc;
-    std::vector<uint32_t> bh(2 * MAX_BIRDIE_INTERVALS, 0);
-    std::copy(birdies.begin(), birdies.end(), bh.begin());
-    HIP_TRY(hipMemsetAsync(c->peak_counts.p, 0, (size_t)nt * sizeof(uint32_t), s));
-    HIP_TRY(hipMemsetAsync(c->counts.p, 0, (size_t)(nt + 1) * sizeof(uint32_t), s));
-    if (ev) HIP_TRY(hipEventRecord(ev[0], s));
-    launch_fft(s, g, sl.series->p, (int)p.fp16, (const float*)sl.mean32->p, (const float*)sl.corr16->p,
-               nt, (float2*)c->work.p, c->spec_t.p, (double*)c->fft_partial.p,
-               (const float2*)c->twiddles.p, (const float*)c->costabs.p);
-    HIP_TRY(hipGetLastError());
-    if (ev) HIP_TRY(hipEventRecord(ev[1], s));
-    launch_whiten(s, g, c->spec_t.p, (int)p.fp16, nt, (const double*)c->fft_partial.p, npart,
-                  c->raw_nat.p, c->blk_sorted.p, (float*)c->floors.p, (float*)c->scales.p);
-    HIP_TRY(hipGetLastError());
-    if (ev) HIP_TRY(hipEventRecord(ev[2], s));
-    launch_harmsum(s, g, p, c->raw_nat.p, (const float*)c->scales.p, bh.data(), nt,
-                   (RawPeak*)c->peaks.p, (uint32_t*)c->peak_counts.p);
-    HIP_TRY(hipGetLastError());
-    if (ev) HIP_TRY(hipEventRecord(ev[3], s));
-    launch_finalize(s, g, p, (const double*)sl.dms->p, nt, (const RawPeak*)c->peaks.p,
-                    (const uint32_t*)c->peak_counts.p, (CandidateOut*)c->cands.p,
-                    (uint32_t)cap_per_trial, (uint32_t*)c->counts.p);
-    HIP_TRY(hipGetLastError());
+    SpecBufs b;
+    b.series = sl.series->p;
+    b.mean32 = (const float*)sl.mean32->p;
+    b.corr16 = (const float*)sl.corr16->p;
+    b.dms = (const double*)sl.dms->p;
+    b.work = (float2*)c->work.p;
+    b.spec_t = c->spec_t.p;
+    b.raw_nat = c->raw_nat.p;
+    b.scales = (float*)c->scales.p;
+    b.blk_sorted = c->blk_sorted.p;
+    b.floors = (float*)c->floors.p;
+    b.fft_partial = (double*)c->fft_partial.p;
+    b.peaks = (RawPeak*)c->peaks.p;
+    b.peak_counts = (uint32_t*)c->peak_counts.p;
+    b.cands = (CandidateOut*)c->cands.p;
+    b.counts = (uint32_t*)c->counts.p;
+    b.twiddles = (const float2*)c->twiddles.p;
+    b.costabs = (const float*)c->costabs.p;
+    if ((rc = enqueue_spectral(g, p, birdies, b, nt, cap_per_trial, s, ev))) return rc;
     HIP_TRY(hipMemcpyAsync(counts, c->counts.p, (size_t)(nt + 1) * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(out, c->cands.p, (size_t)nt * cap_per_trial * sizeof(CandidateOut),

@@ -99,6 +99,8 @@ void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch
                        uint32_t nchans, const int32_t* delays, uint32_t ntrials, uint32_t n,
                        void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
                        double* partial);
+// fill n words with v (clears counters without memset nodes)
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 // FFT + power
 void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
                 const float* mean32, const float* corr16, uint32_t ntrials, float2* work,

@@ -73,6 +73,34 @@ int dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials
 // table (one f32 trial in channel order) into host out[n]
 int apply_table_to_host(psr_ctx* c, psr_obs* o, uint32_t ch_lo, uint32_t nch,
                         const uint64_t* delays, size_t n, float* out);
+// search::periodicity's bin limits and birdie bins for a geometry
+int make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, SearchParams* p,
+                std::vector<uint32_t>* birdie_bins);
+// device buffers of one run of the spectral chain
+struct SpecBufs {
+    const void* series = nullptr;  // [nt][series_pitch] f32 or f16
+    const float* mean32 = nullptr;
+    const float* corr16 = nullptr;
+    const double* dms = nullptr;  // [nt]
+    float2* work = nullptr;       // spectral_work_bytes
+    void* spec_t = nullptr;       // [nt][spec_pitch]
+    void* raw_nat = nullptr;      // [nt][spec_pitch]
+    float* scales = nullptr;      // [nt][nblocks]
+    void* blk_sorted = nullptr;   // [nt][spec_pitch] unless whiten_fused
+    float* floors = nullptr;      // [nt]
+    double* fft_partial = nullptr;  // [nt][m1 == 1 ? 1 : m2 / 2 + 1]
+    RawPeak* peaks = nullptr;     // [nt][PEAK_CAP]
+    uint32_t* peak_counts = nullptr;  // [nt]
+    CandidateOut* cands = nullptr;    // [nt][cap_per_trial]
+    uint32_t* counts = nullptr;       // [nt + 1]
+    const float2* twiddles = nullptr;  // twiddle_table_size(g)
+    const float* costabs = nullptr;    // COSTAB_SIZE
+};
+size_t spectral_work_bytes(const SpecGeom& g, uint32_t nt);
+// FFT, whitening, harmonic sums and ranking on stream s; allocates nothing
+int enqueue_spectral(const SpecGeom& g, const SearchParams& p, const std::vector<uint32_t>& birdies,
+                     const SpecBufs& b, uint32_t nt, size_t cap_per_trial, hipStream_t s,
+                     hipEvent_t* ev);
 // recompute the FP16 storage offset/scale (m0, s0) from the device samples
 int obs_storage_scale(psr_ctx* c, psr_obs* o);
 // bytes per sample of an observation's channel rows

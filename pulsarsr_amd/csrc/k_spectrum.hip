@@ -1018,6 +1018,16 @@ __global__ __launch_bounds__(256) void k_finalize(const RawPeak* __restrict__ pe
     if (lane == 0) counts[trial] = nk;
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
 void launch_finalize(hipStream_t s, const SpecGeom& g, const SearchParams& p, const double* dms,
                      uint32_t ntrials, const RawPeak* peaks, const uint32_t* peak_counts,
                      CandidateOut* out, uint32_t cap_per_trial, uint32_t* counts) {

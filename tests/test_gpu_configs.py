@@ -141,11 +141,25 @@ def check_large(name):
     assert worst_norm <= 5e-3
     # the report's candidate list: top-20 family leaders, FP16 against FP32
     lead16, _, _ = top_families(got16, n, w.tsamp)
-    assert len(lead16) == len(leaders32) > 0
-    for a, b in zip(lead16, leaders32):
-        assert a["dm"] == b["dm"] and a["harmonics"] == b["harmonics"], (a, b)
-        assert abs(a["frequency_hz"] - b["frequency_hz"]) <= 1.5 / (n * w.tsamp)
+    same = compare_leaders(lead16, leaders32, 1.5 / (n * w.tsamp))
+    print("top-20 leaders identical fp16 vs fp32: %d of %d" % (same, len(leaders32)), flush=True)
     obs.close()
+    return same, len(leaders32)
+
+
+def compare_leaders(l16, l32, res, tie=1e-2):
+    """Rank by rank, the FP16 and FP32 family leaders agree in DM, harmonics and
+    frequency (within the grouping resolution), except where two families' scores
+    are within `tie` of each other: FP16 rounding may then swap their order, or
+    which of them takes the last place.  Returns the number of identical ranks."""
+    assert len(l16) == len(l32) > 0
+    same = 0
+    for i, (a, b) in enumerate(zip(l16, l32)):
+        if a["dm"] == b["dm"] and a["harmonics"] == b["harmonics"] and abs(a["frequency_hz"] - b["frequency_hz"]) <= res:
+            same += 1
+            continue
+        assert abs(a["score"] - b["score"]) <= tie * abs(b["score"]), ("leaders differ beyond a near tie", i, a, b)
+    return same
 
 
 def test_precision_workload(ctx):
@@ -153,7 +167,8 @@ def test_precision_workload(ctx):
 
 
 def test_headline_workload(ctx):
-    check_large("headline-1024ch-8bit")
+    same, total = check_large("headline-1024ch-8bit")
+    assert same == total  # the headline's top-20 list is identical in FP16 and FP32
 
 
 def test_lband_workload_all_trials(ctx):
