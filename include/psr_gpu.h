@@ -201,6 +201,19 @@ int psr_last_stage_times(const psr_ctx *ctx, psr_stage_times *out);
 /* search::periodicity on one host series (crates/search/src/lib.rs:33-145), 1:1. */
 int psr_periodicity(psr_ctx *ctx, const float *series, size_t n, double tsamp, double dm,
                     const psr_search_cfg *cfg, psr_candidate *out, size_t cap, size_t *count);
+/* No counterpart in the reference.  psr_periodicity without any device
+ * allocation, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as psr_periodicity; every device buffer comes
+ * from one static per-device scratch array of 4 MiB that the library holds
+ * until the process ends.  Limits: n <= PSR_NOALLOC_MAX_SERIES samples and
+ * cap <= PSR_NOALLOC_MAX_CANDIDATES (larger requests fail with PSR_E_SHAPE /
+ * PSR_E_INVALID).  All calls on one device share that array: calls that overlap
+ * on one device race (serialise them); calls on different devices do not. */
+#define PSR_NOALLOC_MAX_SERIES 131072u
+#define PSR_NOALLOC_MAX_CANDIDATES 256u
+int psr_periodicity_noalloc(psr_ctx *ctx, const float *series, size_t n, double tsamp, double dm,
+                            const psr_search_cfg *cfg, psr_candidate *out, size_t cap, size_t *count);
 /* The spectral stages of periodicity for checks: raw |X|^2 and whitened power
  * (crates/search/src/lib.rs:35-72), each nbins = next_pow2(n)+1 floats.  With
  * cfg->precision = PSR_PRECISION_FP16 the series (less its mean) is stored in

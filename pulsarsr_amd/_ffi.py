@@ -21,6 +21,8 @@ PSR_E_OVERFLOW = -7
 PRECISION_FP32 = 0
 PRECISION_FP16 = 1
 PSR_CTX_OVERLAP = 1
+PSR_NOALLOC_MAX_SERIES = 131072
+PSR_NOALLOC_MAX_CANDIDATES = 256
 
 
 class Candidate(C.Structure):
@@ -144,6 +146,9 @@ EXPORTS = [
     ("psr_search", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_size_t, C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t, C.POINTER(C.c_uint32)]),
     ("psr_last_stage_times", C.c_int, [C.c_void_p, C.POINTER(StageTimes)]),
     ("psr_periodicity", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("psr_periodicity_noalloc", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double,
+                                          C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t,
+                                          C.POINTER(C.c_size_t)]),
     ("psr_spectrum", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(SearchCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
     ("psr_fold", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     ("psr_refine", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),

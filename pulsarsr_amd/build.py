@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libpsrgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["k_dedisperse.hip", "k_fft.hip", "k_spectrum.hip", "k_fold.hip", "k_rfi.hip", "api.cpp",
-           "analyze.cpp", "rfi.cpp", "host_math.cpp"]
+           "analyze.cpp", "rfi.cpp", "host_math.cpp", "noalloc.cpp"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -58,6 +58,22 @@ def periodicity(series, tsamp, dm, config=None, ctx=None):
     return _cands(out, n.value)
 
 
+def periodicity_noalloc(series, tsamp, dm, config=None, ctx=None):
+    """periodicity through psr_periodicity_noalloc: no device allocation (static
+    per-device scratch; series of at most 131072 samples, 256 candidates)."""
+    config = config or Config()
+    ctx = ctx or default_context()
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    cap = max(1, int(config.max_candidates))
+    out = (_ffi.Candidate * cap)()
+    n = C.c_size_t()
+    cfg, keep = config._c()
+    check(_ffi.lib().psr_periodicity_noalloc(ctx.handle, x.ctypes.data_as(C.POINTER(C.c_float)), len(x),
+                                             float(tsamp), float(dm), C.byref(cfg), out, cap, C.byref(n)))
+    del keep
+    return _cands(out, n.value)
+
+
 def spectrum(series, tsamp, config=None, ctx=None):
     """(raw |X|^2, whitened power) of periodicity's spectral stages (lib.rs:35-72)."""
     config = config or Config()
