@@ -222,6 +222,43 @@ int psr_periodicity_noalloc(psr_ctx *ctx, const float *series, size_t n, double 
 int psr_spectrum(psr_ctx *ctx, const float *series, size_t n, double tsamp,
                  const psr_search_cfg *cfg, float *raw, float *white, size_t *nbins);
 
+/* ---- many observations ------------------------------------------------- */
+/* The reference searches one observation per `psr search` run (the per-trial
+ * loop of crates/cli/src/main.rs:462-469) and its many-observation harness
+ * (crates/synth/examples/confirmation.rs) loops over observations in turn.  A
+ * psr_batch runs that per-trial loop for many device-resident observations of
+ * uneven length, channel count, bit depth and DM plan in one device schedule:
+ * per round of trials, one launch per stage (a persistent dedispersion kernel
+ * fed by per-XCD work queues of (observation, DM block, time-tile span) items,
+ * then FFT, whitening, harmonic sums and ranking over all the round's trials).
+ * Every observation gets exactly the candidates psr_search would give it (with
+ * the default shapes: bit for bit; tuned FFT splits round the spectrum
+ * differently). */
+typedef struct psr_batch psr_batch;
+#define PSR_BATCH_GRAPH 1u    /* capture the whole schedule in one hipGraph at creation; runs replay it */
+#define PSR_BATCH_AUTOTUNE 2u /* time candidate shapes on the batch's own data at creation (FFT split
+                               * and round size per FFT length; dedispersion tile span and workgroups) */
+typedef struct {
+    uint32_t flags;         /* PSR_BATCH_* */
+    uint32_t pad_;
+    uint64_t memory_budget; /* device bytes per round; 0 = half the free memory, at most 64 GiB */
+} psr_batch_opts;
+/* Plan a batch over obs[nobs]: observation i searches dms[i][ntrials[i]] with
+ * series length lengths[i] (0 = None, as psr_search; lengths may be NULL),
+ * all with `cfg` and cap_per_trial candidates per trial.  Device buffers, the
+ * tuned shapes and the graph belong to the batch; the observations must
+ * outlive it.  Errors as psr_search. */
+int psr_batch_create(psr_ctx *ctx, psr_obs *const *obs, size_t nobs, const double *const *dms,
+                     const size_t *ntrials, const size_t *lengths, const psr_search_cfg *cfg,
+                     size_t cap_per_trial, const psr_batch_opts *opts, psr_batch **out);
+/* Run the batch: observation i's trial t (its caller order) lands at row
+ * sum_{k<i} ntrials[k] + t of out[rows * cap_per_trial] and counts[rows]. */
+int psr_batch_run(psr_batch *batch, psr_candidate *out, uint32_t *counts);
+/* JSON description: rounds, FFT splits, dedispersion shape, autotuner timings,
+ * the last run's device time (last_ms).  *len = full length; json may be NULL. */
+int psr_batch_info(const psr_batch *batch, char *json, size_t cap, size_t *len);
+void psr_batch_destroy(psr_batch *batch);
+
 /* ---- folding ------------------------------------------------------------ */
 /* fold::profile_at for `nprof` (period, sample_offset) pairs over one host series
  * (crates/fold/src/lib.rs:16-80).  values/counts are [nprof*bins]. */

@@ -21,6 +21,8 @@ PSR_E_OVERFLOW = -7
 PRECISION_FP32 = 0
 PRECISION_FP16 = 1
 PSR_CTX_OVERLAP = 1
+PSR_BATCH_GRAPH = 1
+PSR_BATCH_AUTOTUNE = 2
 PSR_NOALLOC_MAX_SERIES = 131072
 PSR_NOALLOC_MAX_CANDIDATES = 256
 
@@ -105,6 +107,10 @@ class StageTimes(C.Structure):
                 ("fft_len", C.c_uint64), ("launches", C.c_uint64)]
 
 
+class BatchOpts(C.Structure):  # psr_batch_opts
+    _fields_ = [("flags", C.c_uint32), ("pad_", C.c_uint32), ("memory_budget", C.c_uint64)]
+
+
 class SynthPulsar(C.Structure):
     _fields_ = [("period_s", C.c_double), ("dm", C.c_double), ("duty", C.c_double),
                 ("amplitude", C.c_double), ("phase", C.c_double)]
@@ -152,6 +158,12 @@ EXPORTS = [
     ("psr_spectrum", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(SearchCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
     ("psr_fold", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     ("psr_refine", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
+    ("psr_batch_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(C.POINTER(C.c_double)),
+                                   C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(SearchCfg), C.c_size_t,
+                                   C.POINTER(BatchOpts), C.POINTER(C.c_void_p)]),
+    ("psr_batch_run", C.c_int, [C.c_void_p, C.POINTER(Candidate), C.POINTER(C.c_uint32)]),
+    ("psr_batch_info", C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("psr_batch_destroy", None, [C.c_void_p]),
     ("psr_rules_default", None, [C.POINTER(Rules)]),
     ("psr_measure", C.c_int, [C.POINTER(C.c_float), C.c_size_t, C.c_uint64, C.c_double, C.c_double,
                               C.POINTER(Window), C.POINTER(Measurement)]),
@@ -181,6 +193,22 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+_synth_gpu = None
+SYNTH_GPU_PATH = os.path.join(HERE, "libpsrsynthgpu.so")
+
+
+def synth_gpu_lib():
+    global _synth_gpu
+    if _synth_gpu is None:
+        if not os.path.exists(SYNTH_GPU_PATH):
+            raise RuntimeError("libpsrsynthgpu.so is not built (run pulsarsr_amd.build.build())")
+        L = C.CDLL(SYNTH_GPU_PATH)
+        L.psr_synth_generate_gpu.restype = C.c_int
+        L.psr_synth_generate_gpu.argtypes = [C.POINTER(SynthConfig), C.c_void_p, C.c_uint64]
+        _synth_gpu = L
+    return _synth_gpu
 
 
 def synth_lib():

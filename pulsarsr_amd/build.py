@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libpsrgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["k_dedisperse.hip", "k_fft.hip", "k_spectrum.hip", "k_fold.hip", "k_rfi.hip", "api.cpp",
-           "analyze.cpp", "rfi.cpp", "host_math.cpp", "noalloc.cpp"]
+           "analyze.cpp", "rfi.cpp", "host_math.cpp", "noalloc.cpp", "batch.cpp"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -50,6 +50,21 @@ def build_synth():
     return SYNTH_LIB
 
 
+SYNTH_GPU_LIB = os.path.join(HERE, "libpsrsynthgpu.so")
+
+
+def build_synth_gpu():
+    """The GPU workload generator (k_synth.hip), a library of its own: it is a
+    workload tool, not part of the search path."""
+    tmp = SYNTH_GPU_LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-shared", "-x", "hip", os.path.join(CSRC, "k_synth.hip"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("synth gpu build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, SYNTH_GPU_LIB)
+    return SYNTH_GPU_LIB
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -66,6 +81,7 @@ def build(verbose=False):
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
     build_synth()
+    build_synth_gpu()
     return LIB
 
 

@@ -53,12 +53,10 @@ int check_device(psr_ctx* c) {
 
 }  // namespace psr
 
-namespace {
-
 // apply()'s length rules (crates/dedisperse/src/lib.rs:72-78) for the largest DM
 // of the batch; delays are non-decreasing in DM so it bounds every trial.
-int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t length,
-                   size_t* n_out, uint64_t* maxdelay_out, bool fft_limit = true) {
+int psr::resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t length,
+                        size_t* n_out, uint64_t* maxdelay_out, bool fft_limit) {
     double dmax = 0.0;
     for (size_t i = 0; i < ntrials; i++) {
         if (!std::isfinite(dms[i]) || dms[i] < 0.0)
@@ -83,13 +81,10 @@ int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t l
     return PSR_OK;
 }
 
-// Upper bound of the within-block delay spread of any channel: delays are
-// round(K*dm*a_c/tsamp) with a_c <= a_max, so the spread over a block of DMs is
-// at most K*a_max*(dm_hi - dm_lo)/tsamp + 1; two samples of slack on top.
 // Per-channel bound on the delay spread (samples) over any block of `block`
 // consecutive sorted trials: the widest DM span of a block times the channel's
 // dispersion slope, plus 3 samples for rounding.
-std::vector<uint32_t> spread_bounds(const psr_obs* o, const double* dms_sorted, size_t ntrials,
+std::vector<uint32_t> psr::spread_bounds(const psr_obs* o, const double* dms_sorted, size_t ntrials,
                                     size_t block) {
     double worst = 0.0;
     for (size_t b = 0; b < ntrials; b += block) {
@@ -104,6 +99,8 @@ std::vector<uint32_t> spread_bounds(const psr_obs* o, const double* dms_sorted, 
     }
     return out;
 }
+
+namespace {
 
 int upload_twiddles(psr_ctx* c, const SpecGeom& g) {
     if (!c->costabs.p) {

@@ -94,6 +94,44 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
                       double* partial, const uint32_t* spread);
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials);
+
+// One observation's share (a range of its DM trials) of a many-observation
+// dedispersion launch.  Host fills every field; device pointers point into the
+// batch's buffers.
+struct DedispJob {
+    const uint8_t* data;   // u8 channel rows of the observation
+    uint64_t pitch;        // bytes between channel rows
+    const double* dms;     // [ntrials], ascending
+    const double* inv_f2;  // [nchans] 1/(f*f)
+    double ref_inv, tsamp;
+    int32_t* delays;       // [ntrials][nchans]
+    int32_t* bases;        // [nblk][nchans]
+    uint32_t* plan;        // [nblk][plan_words]
+    uint64_t plan_words;
+    void* out;             // series row of the job's first trial
+    double* partial;       // [ntrials][ntiles]
+    uint32_t nchans, ntrials, nblk, ntiles, n, out_pitch;
+    uint32_t phase_slots, RB, kmax, windows;
+    float m0, inv_s0;
+    uint32_t entry0, blk0, item0, trial0;  // first delay entry, plan block, work item, trial in the batch
+    double m0d, inv_s0d;                   // FP16 storage offset and 1/scale in f64 (trial means)
+};
+// Fills RB, windows, kmax, phase_slots and plan_words for a job with nchans
+// u8 rows of `pitch` bytes, samples <= maxval and per-channel delay spreads
+// `spread` over any 64-trial block; false when the LDS-staged kernel cannot
+// take it (the caller then dedisperses it with launch_dedisp_f32).
+bool dedisp_job_setup(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uint32_t* spread,
+                      DedispJob* j);
+uint32_t dedisp_tiles(uint32_t n);  // 512-output time tiles of a series
+size_t dedisp_plan_words(size_t nchans);  // plan words per 64-trial block
+// delays, plans and the persistent work-queue dedispersion of a batch of jobs:
+// entry0/blk0/item0 hold each job's first delay entry / plan block / work item
+// (njobs ascending values starting at 0); counters = 8 device words.
+void launch_dedisp_batch(hipStream_t s, const DedispJob* jobs, const uint32_t* entry0,
+                         const uint32_t* blk0, const uint32_t* item0, uint32_t njobs,
+                         uint32_t total_entries, uint32_t total_blocks, uint32_t total_items,
+                         uint32_t span, int fp16, uint32_t max_nchans, uint32_t grid,
+                         uint32_t* counters);
 // sequential-order f32 dedispersion (16-bit and float rows)
 void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch,
                        uint32_t nchans, const int32_t* delays, uint32_t ntrials, uint32_t n,
@@ -102,9 +140,13 @@ void launch_dedisp_f32(hipStream_t s, const void* data, int kind, uint64_t pitch
 // fill n words with v (clears counters without memset nodes)
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 // FFT + power
+// trial_n (optional): per-trial series lengths when the trials of one launch
+// come from several observations (the geometry's n otherwise); batch_trials:
+// trials per pass-1/pass-2 round (0 = fft_work_trials)
 void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
                 const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
-                void* spec_t, double* partial_sums, const float2* tw, const float* cos_tables);
+                void* spec_t, double* partial_sums, const float2* tw, const float* cos_tables,
+                const uint32_t* trial_n = nullptr, uint32_t batch_trials = 0);
 
 // Quarter-wave cosine tables C_L[j] = cos(2 pi j / L), j = 0..L/4, for every
 // power of two L = 2..COSTAB_MAX, concatenated in increasing L.
@@ -133,10 +175,25 @@ void launch_whiten(hipStream_t s, const SpecGeom& g, const void* spec_t, int fp1
 void launch_harmsum(hipStream_t s, const SpecGeom& g, const SearchParams& p, const void* raw_nat,
                     const float* scales, const uint32_t* birdies, uint32_t ntrials,
                     RawPeak* peaks, uint32_t* peak_counts);
+// harmonic sums over trials of several observations: `jobs` holds
+// harm_args_bytes() per observation (fill_harm_args), trial_job[t] the
+// observation of trial t, max_tiles the largest harmsum_tiles() among them
+size_t harm_args_bytes();
+void fill_harm_args(const SearchParams& p, const uint32_t* birdies, void* out);
+uint32_t harmsum_tiles(const SearchParams& p);
+void launch_harmsum_multi(hipStream_t s, const SpecGeom& g, int fp16, uint32_t max_tiles, const void* raw_nat,
+                          const float* scales, const void* jobs, const uint32_t* trial_job, uint32_t ntrials,
+                          RawPeak* peaks, uint32_t* peak_counts);
+void launch_finalize_multi(hipStream_t s, const double* dms, uint32_t ntrials, const RawPeak* peaks,
+                           const uint32_t* peak_counts, const void* jobs, const uint32_t* trial_job,
+                           CandidateOut* out, uint32_t cap_per_trial, uint32_t* counts);
 // rank, dedup, cap
 void launch_finalize(hipStream_t s, const SpecGeom& g, const SearchParams& p, const double* dms,
                      uint32_t ntrials, const RawPeak* peaks, const uint32_t* peak_counts,
                      CandidateOut* out, uint32_t cap_per_trial, uint32_t* counts);
+// trial means of a batch of jobs: trial_job[t] = job of batch trial t
+void launch_trial_mean_multi(hipStream_t s, const DedispJob* jobs, const uint32_t* trial_job,
+                             uint32_t ntrials, float* mean32, float* corr16);
 void launch_trial_mean(hipStream_t s, const double* partial, uint32_t ntiles, uint32_t ntrials,
                        uint32_t n, double m0, double inv_s0, float* mean32, float* corr16);
 // folding
@@ -169,5 +226,8 @@ void launch_dm_response(hipStream_t s, const float* values, const double* means,
 size_t twiddle_table_size(const SpecGeom& g);
 void fill_twiddles(const SpecGeom& g, float2* host_table);
 SpecGeom make_geometry(uint32_t n);
+// the same with the four-step split's pass-1 length m2 chosen by the caller
+// (ignored for one-pass sizes); check fft_shape_supported() on the result
+SpecGeom make_geometry_split(uint32_t n, uint32_t m2);
 
 }  // namespace psr

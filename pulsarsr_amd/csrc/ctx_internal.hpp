@@ -73,6 +73,13 @@ int dedisperse_to_host(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials
 // table (one f32 trial in channel order) into host out[n]
 int apply_table_to_host(psr_ctx* c, psr_obs* o, uint32_t ch_lo, uint32_t nch,
                         const uint64_t* delays, size_t n, float* out);
+// apply()'s length rules over a set of DMs: output length n (length 0 = None)
+// and the largest delay; fft_limit rejects series a search cannot take (> 2^23)
+int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t length, size_t* n_out,
+                   uint64_t* maxdelay_out, bool fft_limit = true);
+// per-channel delay-spread bound over any `block` consecutive sorted trials
+std::vector<uint32_t> spread_bounds(const psr_obs* o, const double* dms_sorted, size_t ntrials,
+                                    size_t block);
 // search::periodicity's bin limits and birdie bins for a geometry
 int make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, SearchParams* p,
                 std::vector<uint32_t>* birdie_bins);

@@ -234,6 +234,15 @@ static uint32_t next_pow2(uint32_t n) {
     return p;
 }
 
+SpecGeom make_geometry_split(uint32_t n, uint32_t m2) {
+    SpecGeom g = make_geometry(n);
+    if (g.m1 > 1 && m2 && g.m % m2 == 0) {
+        g.m2 = m2;
+        g.m1 = g.m / m2;
+    }
+    return g;
+}
+
 SpecGeom make_geometry(uint32_t n) {
     SpecGeom g{};
     g.n = n;

@@ -225,10 +225,11 @@ __device__ __forceinline__ uint32_t ddp_mis(int32_t base) { return (uint32_t)(ba
 // per slot its stage | position << 16, then per channel pair its kind (K pair
 // rows, 0 = single-channel rows) and first slot, and per row its delta,
 // L | W << 16 and LDS offset.
-__global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ delays, uint32_t nchans,
-                                                 uint32_t ntrials, uint32_t kmax, uint32_t max_windows,
-                                                 uint32_t row_budget, int32_t* __restrict__ bases,
-                                                 uint32_t* __restrict__ plan, uint64_t plan_words) {
+__device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays, uint32_t nchans,
+                                              uint32_t ntrials, uint32_t kmax, uint32_t max_windows,
+                                              uint32_t row_budget, int32_t* __restrict__ bases,
+                                              uint32_t* __restrict__ plan, uint64_t plan_words,
+                                              uint32_t blk) {
     extern __shared__ int32_t sbase[];
     const uint32_t npair = (nchans + 1) / 2;
     uint32_t* sloc = (uint32_t*)(sbase + nchans);
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
     int32_t* udel = (int32_t*)(uslot + npair);
     uint32_t* uLW = (uint32_t*)(udel + (size_t)npair * DDP_KMAX);
     uint32_t* uRO = uLW + (size_t)npair * DDP_KMAX;
-    const uint32_t blk = blockIdx.x, d0 = blk * DD_TRIALS;
+    const uint32_t d0 = blk * DD_TRIALS;
     uint32_t* pb = plan + (size_t)blk * plan_words;
     auto delay = [&](uint32_t m, uint32_t c) {
         return delays[(size_t)min(d0 + m, ntrials - 1) * nchans + c];
@@ -421,11 +422,25 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
 }
 
 
-__global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
-    const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_slots,
-    const int32_t* __restrict__ bases, const uint32_t* __restrict__ plan, uint64_t plan_words,
-    uint32_t ntrials, uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out,
-    uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t RB) {
+__global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ delays, uint32_t nchans,
+                                                 uint32_t ntrials, uint32_t kmax, uint32_t max_windows,
+                                                 uint32_t row_budget, int32_t* __restrict__ bases,
+                                                 uint32_t* __restrict__ plan, uint64_t plan_words) {
+    dd_plan_block(delays, nchans, ntrials, kmax, max_windows, row_budget, bases, plan, plan_words,
+                  blockIdx.x);
+}
+
+// One work item of the u8 dedispersion: DM block `blk` (trials blk * 64 ..) over
+// time tile `tile` (outputs tile * 512 ..).  bases/plan are the whole launch's
+// tables (indexed by blk), partial has ntiles words per trial.
+__device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64_t pitch,
+                                        uint32_t nchans, uint32_t phase_slots,
+                                        const int32_t* __restrict__ bases,
+                                        const uint32_t* __restrict__ plan, uint64_t plan_words,
+                                        uint32_t ntrials, uint32_t ntiles, uint32_t n,
+                                        void* __restrict__ out, uint32_t out_pitch, int fp16, float m0,
+                                        float inv_s0, double* __restrict__ partial, uint32_t RB,
+                                        uint32_t tile, uint32_t blk) {
     // Static LDS: the stage's rows, three stage records (the current one, the
     // next, whose raw windows are loaded during this stage, and the one after,
     // loaded now) and the raw windows [window][RB]; records and windows arrive
@@ -435,13 +450,6 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     __shared__ __align__(16) uint32_t srec[3][DDP_STAGE_PITCH];
     __shared__ __align__(16) uint8_t raw[DD_RAW_BYTES];
     extern __shared__ __align__(16) int32_t lbase[];
-
-    const uint32_t total = ntiles * nblk;
-    const uint32_t id = blockIdx.x;
-    const uint32_t xcd = id % DD_XCDS, local = id / DD_XCDS;
-    const uint32_t q = total / DD_XCDS, r = total % DD_XCDS;
-    const uint32_t work = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
-    const uint32_t tile = work / nblk, blk = work % nblk;
 
     const uint32_t t0 = tile * DD_TILE;
     const uint32_t d0 = blk * DD_TRIALS;
@@ -687,6 +695,23 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     }
 }
 
+// One launch over the (tile, block) items of one observation; workgroup ids
+// are remapped XCD-aware (each XCD walks a contiguous run of items, block
+// fastest, so a tile's channel data is reused from that XCD's L2).
+__global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
+    const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_slots,
+    const int32_t* __restrict__ bases, const uint32_t* __restrict__ plan, uint64_t plan_words,
+    uint32_t ntrials, uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out,
+    uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t RB) {
+    const uint32_t total = ntiles * nblk;
+    const uint32_t id = blockIdx.x;
+    const uint32_t xcd = id % DD_XCDS, local = id / DD_XCDS;
+    const uint32_t q = total / DD_XCDS, r = total % DD_XCDS;
+    const uint32_t work = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
+    dd_item(data, pitch, nchans, phase_slots, bases, plan, plan_words, ntrials, ntiles, n, out, out_pitch,
+            fp16, m0, inv_s0, partial, RB, work / nblk, work % nblk);
+}
+
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
     return nblk * nchans + nblk * ddp_plan_words(nchans);  // bases, then the plans
@@ -740,6 +765,145 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
                        m0, inv_s0, partial, RB);
     return true;
 }
+
+// ------------------------------------------- many observations per launch
+// Index of the job whose range [first[j], first[j + 1]) holds i (first[0] = 0,
+// ascending; njobs entries): a binary search over scalar-loaded offsets.
+__device__ __forceinline__ uint32_t job_of(const uint32_t* __restrict__ first, uint32_t njobs, uint32_t i) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Delay tables of every job: entry e of job j is trial e / nchans, channel
+// e % nchans, computed exactly as k_delays.
+__global__ __launch_bounds__(256) void k_delays_multi(const DedispJob* __restrict__ jobs,
+                                                      const uint32_t* __restrict__ entry0, uint32_t njobs,
+                                                      uint32_t total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const DedispJob& j = jobs[job_of(entry0, njobs, i)];
+    const uint32_t e = i - j.entry0;
+    const uint32_t d = e / j.nchans, c = e % j.nchans;
+    const double k_dm = 4148.808 * j.dms[d];
+    const double samples = round(k_dm * (j.inv_f2[c] - j.ref_inv) / j.tsamp);
+    j.delays[e] = (int32_t)fmax(samples, 0.0);
+}
+
+// Plans of every job's DM blocks: one workgroup per block of the batch.
+__global__ __launch_bounds__(256) void k_dd_plan_multi(const DedispJob* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ blk0, uint32_t njobs) {
+    const DedispJob& j = jobs[job_of(blk0, njobs, blockIdx.x)];
+    dd_plan_block(j.delays, j.nchans, j.ntrials, j.kmax, j.windows, (uint32_t)DD_ROW_BYTES, j.bases, j.plan,
+                  j.plan_words, blockIdx.x - j.blk0);
+}
+
+// Persistent dedispersion over the work items of many observations: item i of
+// job j covers DM block (i - item0) % nblk and `span` consecutive time tiles
+// starting at span * ((i - item0) / nblk).  The items are split into one
+// contiguous queue per XCD (queue x: items [x Q, (x + 1) Q)); a workgroup pops
+// items from its own XCD's queue (workgroups are dispatched round-robin over
+// the XCDs, so blockIdx.x % 8 is its XCD) and, once that queue is empty,
+// steals from the next ones, so uneven observations balance at the end.
+// Every workgroup exits once all queues are empty.
+__global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_queue(const DedispJob* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ item0,
+                                                               uint32_t njobs, uint32_t total,
+                                                               uint32_t span, int fp16,
+                                                               uint32_t* __restrict__ counters) {
+    __shared__ uint32_t s_item;
+    const uint32_t home = blockIdx.x % DD_XCDS;
+    const uint32_t Q = (total + DD_XCDS - 1) / DD_XCDS;
+    uint32_t tried = 0;  // thread 0: queues found empty so far
+    for (;;) {
+        __syncthreads();  // the previous item is done with every LDS buffer
+        if (threadIdx.x == 0) {
+            uint32_t it = 0xffffffffu;
+            while (tried < (uint32_t)DD_XCDS) {
+                const uint32_t qq = (home + tried) % DD_XCDS;
+                const uint32_t beg = qq * Q, end = min(beg + Q, total);
+                const uint32_t v = beg < end ? atomicAdd(&counters[qq], 1u) : end - beg;
+                if (v < end - beg) {
+                    it = beg + v;
+                    break;
+                }
+                tried++;
+            }
+            s_item = it;
+        }
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item == 0xffffffffu) break;
+        const DedispJob& j = jobs[job_of(item0, njobs, item)];
+        const uint32_t local = item - j.item0;
+        const uint32_t blk = local % j.nblk, tg = local / j.nblk;
+        for (uint32_t k = 0; k < span; k++) {
+            const uint32_t tile = tg * span + k;
+            if (tile >= j.ntiles) break;
+            if (k) __syncthreads();
+            dd_item(j.data, j.pitch, j.nchans, j.phase_slots, j.bases, j.plan, j.plan_words, j.ntrials,
+                    j.ntiles, j.n, j.out, j.out_pitch, fp16, j.m0, j.inv_s0, j.partial, j.RB, tile, blk);
+        }
+    }
+}
+
+void launch_dedisp_batch(hipStream_t s, const DedispJob* jobs, const uint32_t* entry0,
+                         const uint32_t* blk0, const uint32_t* item0, uint32_t njobs,
+                         uint32_t total_entries, uint32_t total_blocks, uint32_t total_items,
+                         uint32_t span, int fp16, uint32_t max_nchans, uint32_t grid,
+                         uint32_t* counters) {
+    if (total_entries)
+        hipLaunchKernelGGL(k_delays_multi, dim3((total_entries + 255) / 256), dim3(256), 0, s, jobs, entry0,
+                           njobs, total_entries);
+    if (!total_blocks || !total_items || !grid) return;
+    const size_t npair = (max_nchans + 1) / 2;
+    const size_t plan_shm = 4 * (2 * (size_t)max_nchans + 2 * npair + 3 * DDP_KMAX * npair);
+    if (plan_shm > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dd_plan_multi),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan_shm);
+    hipLaunchKernelGGL(k_dd_plan_multi, dim3(total_blocks), dim3(256), plan_shm, s, jobs, blk0, njobs);
+    launch_fill_u32(s, counters, DD_XCDS, 0u);
+    const size_t shm = 4 * (size_t)max_nchans;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dedisp_queue),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(k_dedisp_queue, dim3(grid), dim3(DD_THREADS), shm, s, jobs, item0, njobs,
+                       total_items, span, fp16, counters);
+}
+
+bool dedisp_job_setup(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uint32_t* spread,
+                      DedispJob* j) {
+    if (nchans == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
+    if (pitch % 16 || pitch < 16 || pitch >= (1ull << 31) || nchans > 0xffff) return false;
+    uint32_t smax = 0;
+    for (uint32_t c = 0; c < nchans; c++) smax = std::max(smax, spread[c]);
+    const uint64_t wmax = (uint64_t)smax + 6 + DD_TILE / 8 + 3;
+    const uint32_t RB = (uint32_t)(((uint64_t)smax + 600 + 1023) / 1024 * 1024);
+    if (RB > DD_RAW_BYTES) return false;
+    const uint32_t windows = std::min<uint32_t>(DDP_MAX_WINDOWS, DD_RAW_BYTES / RB);
+    if (windows < 2 || 16 * wmax > (uint64_t)DD_ROW_BYTES) return false;
+    uint32_t kmax = DDP_KMAX;
+    while (kmax > 1 && (uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax--;
+    if ((uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax = 0;
+    const size_t npair = (nchans + 1) / 2;
+    if (4 * (2 * (size_t)nchans + 2 * npair + 3 * DDP_KMAX * npair) > 160 * 1024) return false;
+    if (4 * (size_t)nchans > 64 * 1024) return false;
+    uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
+    phase = phase > 4096 ? 4096 : phase;
+    if (phase < 2) return false;
+    j->RB = RB;
+    j->windows = windows;
+    j->kmax = kmax;
+    j->phase_slots = phase;
+    j->plan_words = ddp_plan_words(nchans);
+    return true;
+}
+
+uint32_t dedisp_tiles(uint32_t n) { return (n + DD_TILE - 1) / DD_TILE; }
+size_t dedisp_plan_words(size_t nchans) { return ddp_plan_words((uint32_t)nchans); }
 
 // ---------------------------------------- channel-ordered f32 dedispersion
 // For 16-bit and float samples (and 8-bit ones whose delay spread is too wide

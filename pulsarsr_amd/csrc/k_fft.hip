@@ -366,7 +366,8 @@ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 template <int L, bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_single(const void* __restrict__ series,
                                                     const float* __restrict__ mean32,
-                                                    const float* __restrict__ corr16, uint32_t n,
+                                                    const float* __restrict__ corr16, uint32_t n_all,
+                                                    const uint32_t* __restrict__ trial_n,
                                                     uint32_t series_pitch,
                                                     void* __restrict__ spec, uint64_t spec_pitch,
                                                     double* __restrict__ partial,
@@ -382,6 +383,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_single(const void* __restrict__ 
     const float* CN = cos_tables + costab_offset(2 * M);  // length-2M table (r2c twiddles)
     __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.x;
+    const uint32_t n = trial_n ? trial_n[trial] : n_all;  // per-trial length (many-observation batches)
     const size_t row = (size_t)trial * series_pitch;
     const float mean = F16 ? 0.f : mean32[trial];
     const float corr = F16 ? corr16[trial] : 0.f;
@@ -459,7 +461,8 @@ __device__ __forceinline__ float swz16(float x) {  // value of lane l ^ 16 (with
 template <bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ series,
                                                     const float* __restrict__ mean32,
-                                                    const float* __restrict__ corr16, uint32_t n,
+                                                    const float* __restrict__ corr16, uint32_t n_all,
+                                                    const uint32_t* __restrict__ trial_n,
                                                     uint32_t series_pitch, uint32_t M, uint32_t m1,
                                                     float2* __restrict__ work,
                                                     const float2* __restrict__ tw4,
@@ -470,6 +473,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ 
     __shared__ float2 TJ[16 * 32];  // W_512^(j k) at k * 32 + j
     const uint2 wb = xcd_block();
     const uint32_t trial = wb.y, c0 = wb.x * C;
+    const uint32_t n = trial_n ? trial_n[trial] : n_all;
     const size_t row = (size_t)trial * series_pitch;
     const float mean = F16 ? 0.f : mean32[trial];
     const float corr = F16 ? corr16[trial] : 0.f;
@@ -531,7 +535,8 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ 
 template <int M2, bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ series,
                                                    const float* __restrict__ mean32,
-                                                   const float* __restrict__ corr16, uint32_t n,
+                                                   const float* __restrict__ corr16, uint32_t n_all,
+                                                    const uint32_t* __restrict__ trial_n,
                                                    uint32_t series_pitch, uint32_t M, uint32_t m1,
                                                    float2* __restrict__ work,
                                                    const float2* __restrict__ tw4,
@@ -545,6 +550,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ s
     float2* T4 = T1 + NT;
     const uint2 wb = xcd_block();  // column groups sharing cache lines on one XCD
     const uint32_t trial = wb.y;
+    const uint32_t n = trial_n ? trial_n[trial] : n_all;
     const uint32_t c0 = wb.x * C;
     const size_t row = (size_t)trial * series_pitch;
     const float mean = F16 ? 0.f : mean32[trial];
@@ -585,7 +591,8 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ s
 
 // --------------------------------------------------------------- pass 2
 template <int M1, bool F16>
-__global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__ work, uint32_t n,
+__global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__ work, uint32_t n_all,
+                                                   const uint32_t* __restrict__ trial_n,
                                                    uint32_t M, uint32_t m2,
                                                    void* __restrict__ spec, uint64_t spec_pitch,
                                                    double* __restrict__ partial,
@@ -602,6 +609,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__
     const float* C2 = cos_tables + costab_offset(2 * M1);  // W_N^(m2 k1) = W_(2 M1)^k1
     __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.y;
+    const uint32_t n = trial_n ? trial_n[trial] : n_all;
     const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
     const float2* w = work + (size_t)trial * M;
     float2 va[PER], vb[PER];
@@ -667,7 +675,8 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__
 // l ^ 16, which a swizzle reaches.  Row 0 (its own partner at 4096 - k)
 // takes one more LDS exchange.
 template <bool F16>
-__global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict__ work, uint32_t n,
+__global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict__ work, uint32_t n_all,
+                                                   const uint32_t* __restrict__ trial_n,
                                                     uint32_t M, uint32_t m2,
                                                     void* __restrict__ spec, uint64_t spec_pitch,
                                                     double* __restrict__ partial,
@@ -678,6 +687,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
     __shared__ float2 TA[16 * 16];  // W_256^(a k) at k * 16 + a
     __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.y;
+    const uint32_t n = trial_n ? trial_n[trial] : n_all;
     const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
     const float2* w = work + (size_t)trial * M;
     const int t = threadIdx.x;
@@ -764,36 +774,37 @@ static void allow_lds(K kernel, size_t bytes) {
 
 template <int L, bool F16>
 static void launch_single(hipStream_t s, const SpecGeom& g, const void* series,
-                          const float* mean32, const float* corr16, uint32_t ntrials, void* spec,
-                          double* partial, const float* cos_tables) {
+                          const float* mean32, const float* corr16, const uint32_t* trial_n,
+                          uint32_t ntrials, void* spec, double* partial, const float* cos_tables) {
     const size_t shm = (size_t)(L + (L >> 4) + 1 + 2 * (tw_total(L) + 1)) * sizeof(float2);
     allow_lds(k_fft_single<L, F16>, shm);
     hipLaunchKernelGGL((k_fft_single<L, F16>), dim3(ntrials), dim3(FFT_THREADS), shm, s, series,
-                       mean32, corr16, g.n, g.series_pitch, spec, g.spec_pitch, partial, cos_tables);
+                       mean32, corr16, g.n, trial_n, g.series_pitch, spec, g.spec_pitch, partial, cos_tables);
 }
 
 template <int M1, bool F16>
-static void launch_pass2(hipStream_t s, const SpecGeom& g, uint32_t ntrials, const float2* work,
-                         void* spec, double* partial, const float2* twrow, const float* cos_tables) {
+static void launch_pass2(hipStream_t s, const SpecGeom& g, const uint32_t* trial_n, uint32_t ntrials,
+                         const float2* work, void* spec, double* partial, const float2* twrow,
+                         const float* cos_tables) {
     if constexpr (M1 == 4096) {
         hipLaunchKernelGGL((k_fft_pass2r<F16>), dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), 0,
-                           s, work, g.n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
+                           s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
         return;
     }
     const size_t shm =
         (size_t)(2 * M1 + ((2 * M1) >> 4) + 1 + 2 * (tw_total(M1) + 1)) * sizeof(float2);
     allow_lds(k_fft_pass2<M1, F16>, shm);
     hipLaunchKernelGGL((k_fft_pass2<M1, F16>), dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), shm,
-                       s, work, g.n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
+                       s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
 }
 
 template <int M2, bool F16>
 static void launch_pass1(hipStream_t s, const SpecGeom& g, const void* series, const float* mean32,
-                         const float* corr16, uint32_t ntrials, float2* work, const float2* tw4,
-                         const float* cos_tables) {
+                         const float* corr16, const uint32_t* trial_n, uint32_t ntrials, float2* work,
+                         const float2* tw4, const float* cos_tables) {
     if constexpr (M2 == 512) {  // m1 = M / 512 >= 16 columns
         hipLaunchKernelGGL((k_fft_pass1r<F16>), dim3(g.m1 / 16, ntrials), dim3(FFT_THREADS), 0, s,
-                           series, mean32, corr16, g.n, g.series_pitch, g.m, g.m1, work, tw4,
+                           series, mean32, corr16, g.n, trial_n, g.series_pitch, g.m, g.m1, work, tw4,
                            cos_tables);
         return;
     }
@@ -801,19 +812,20 @@ static void launch_pass1(hipStream_t s, const SpecGeom& g, const void* series, c
     const size_t shm = (size_t)(C * M2 + 2 * (tw_total(M2) + 1)) * sizeof(float2);
     allow_lds(k_fft_pass1<M2, F16>, shm);
     hipLaunchKernelGGL((k_fft_pass1<M2, F16>), dim3(g.m1 / C, ntrials), dim3(FFT_THREADS), shm, s,
-                       series, mean32, corr16, g.n, g.series_pitch, g.m, g.m1, work, tw4,
+                       series, mean32, corr16, g.n, trial_n, g.series_pitch, g.m, g.m1, work, tw4,
                        cos_tables);
 }
 
 template <bool F16>
 static void launch_fft_t(hipStream_t s, const SpecGeom& g, const void* series, const float* mean32,
-                         const float* corr16, uint32_t ntrials, float2* work, void* spec,
-                         double* partial, const float2* tw, const float* cos_tables) {
+                         const float* corr16, const uint32_t* trial_n, uint32_t ntrials, float2* work,
+                         void* spec, double* partial, const float2* tw, const float* cos_tables,
+                         uint32_t batch_trials) {
     if (g.m1 == 1) {
         switch (g.m) {
 #define SINGLE(L)                                                                             \
     case L:                                                                                   \
-        launch_single<L, F16>(s, g, series, mean32, corr16, ntrials, spec, partial, cos_tables); \
+        launch_single<L, F16>(s, g, series, mean32, corr16, trial_n, ntrials, spec, partial, cos_tables); \
         break;
             SINGLE(2) SINGLE(4) SINGLE(8) SINGLE(16) SINGLE(32) SINGLE(64) SINGLE(128)
             SINGLE(256) SINGLE(512) SINGLE(1024) SINGLE(2048) SINGLE(4096)
@@ -826,7 +838,7 @@ static void launch_fft_t(hipStream_t s, const SpecGeom& g, const void* series, c
     const float2* twrow = tw + 2048 + g.m / 2048;
     // Trials go through both passes in batches of FFT_BATCH_BYTES of
     // intermediate, which bounds the work buffer independently of the chunk.
-    const uint32_t tb = fft_work_trials(g, ntrials);
+    const uint32_t tb = batch_trials ? (batch_trials < ntrials ? batch_trials : ntrials) : fft_work_trials(g, ntrials);
     const size_t es = F16 ? 2 : 4;
     const uint32_t npart = g.m2 / 2 + 1;
     for (uint32_t b = 0; b < ntrials; b += tb) {
@@ -834,20 +846,21 @@ static void launch_fft_t(hipStream_t s, const SpecGeom& g, const void* series, c
         const void* ser = (const char*)series + (size_t)b * g.series_pitch * es;
         void* sp = (char*)spec + (size_t)b * g.spec_pitch * es;
         double* part = partial + (size_t)b * npart;
+        const uint32_t* tn = trial_n ? trial_n + b : nullptr;
         switch (g.m2) {  // fft_shape_supported() admits exactly these
             case 512:
-                launch_pass1<512, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+                launch_pass1<512, F16>(s, g, ser, mean32 + b, corr16 + b, tn, nb, work, tw4, cos_tables);
                 break;
             case 1024:
-                launch_pass1<1024, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+                launch_pass1<1024, F16>(s, g, ser, mean32 + b, corr16 + b, tn, nb, work, tw4, cos_tables);
                 break;
             case 2048:
-                launch_pass1<2048, F16>(s, g, ser, mean32 + b, corr16 + b, nb, work, tw4, cos_tables);
+                launch_pass1<2048, F16>(s, g, ser, mean32 + b, corr16 + b, tn, nb, work, tw4, cos_tables);
                 break;
         }
         switch (g.m1) {
 #define PASS2(L) \
-    case L: launch_pass2<L, F16>(s, g, nb, work, sp, part, twrow, cos_tables); break;
+    case L: launch_pass2<L, F16>(s, g, tn, nb, work, sp, part, twrow, cos_tables); break;
             PASS2(8) PASS2(16) PASS2(32) PASS2(64) PASS2(128) PASS2(256) PASS2(512) PASS2(1024)
             PASS2(2048) PASS2(4096)
 #undef PASS2
@@ -872,11 +885,14 @@ uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials) {
 
 void launch_fft(hipStream_t s, const SpecGeom& g, const void* series, int fp16,
                 const float* mean32, const float* corr16, uint32_t ntrials, float2* work,
-                void* spec, double* partial, const float2* tw, const float* cos_tables) {
+                void* spec, double* partial, const float2* tw, const float* cos_tables,
+                const uint32_t* trial_n, uint32_t batch_trials) {
     if (fp16)
-        launch_fft_t<true>(s, g, series, mean32, corr16, ntrials, work, spec, partial, tw, cos_tables);
+        launch_fft_t<true>(s, g, series, mean32, corr16, trial_n, ntrials, work, spec, partial, tw, cos_tables,
+                           batch_trials);
     else
-        launch_fft_t<false>(s, g, series, mean32, corr16, ntrials, work, spec, partial, tw, cos_tables);
+        launch_fft_t<false>(s, g, series, mean32, corr16, trial_n, ntrials, work, spec, partial, tw, cos_tables,
+                            batch_trials);
 }
 
 }  // namespace psr

@@ -168,6 +168,32 @@ __global__ __launch_bounds__(256) void k_trial_mean(const double* __restrict__ p
     }
 }
 
+// The same for the trials of many observations (one job each range of trials).
+__global__ __launch_bounds__(256) void k_trial_mean_multi(const DedispJob* __restrict__ jobs,
+                                                          const uint32_t* __restrict__ trial_job,
+                                                          uint32_t ntrials, float* __restrict__ mean32,
+                                                          float* __restrict__ corr16) {
+    const uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= ntrials) return;
+    const DedispJob& j = jobs[trial_job[t]];
+    const uint32_t local = t - j.trial0;
+    double s = 0.0;
+    for (uint32_t i = lane; i < j.ntiles; i += 64) s += j.partial[(size_t)local * j.ntiles + i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) {
+        const double mean = s / (double)j.n;
+        mean32[t] = (float)mean;
+        corr16[t] = (float)((j.m0d - mean) * j.inv_s0d);
+    }
+}
+
+void launch_trial_mean_multi(hipStream_t s, const DedispJob* jobs, const uint32_t* trial_job,
+                             uint32_t ntrials, float* mean32, float* corr16) {
+    hipLaunchKernelGGL(k_trial_mean_multi, dim3((ntrials + 3) / 4), dim3(256), 0, s, jobs, trial_job, ntrials,
+                       mean32, corr16);
+}
+
 void launch_trial_mean(hipStream_t s, const double* partial, uint32_t ntiles, uint32_t ntrials,
                        uint32_t n, double m0, double inv_s0, float* mean32, float* corr16) {
     hipLaunchKernelGGL(k_trial_mean, dim3((ntrials + 3) / 4), dim3(256), 0, s, partial, ntiles,

@@ -14,6 +14,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <type_traits>
 
 #include "common.hpp"
@@ -756,13 +758,17 @@ constexpr int HS_TILES = 2;       // tiles of HS_OUT bins per workgroup, the nex
 // per listed bin then adds its harmonics in order j = 2, 3, ... in f64, as the
 // reference does.  The small list keeps the LDS footprint low enough for eight
 // workgroups per CU.
-template <bool F16>
+// MULTI: the trials of one launch come from several observations; trial t
+// takes the bin limits, levels and birdies of jobs[trial_job[t]] (a otherwise).
+template <bool F16, bool MULTI>
 __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
                                                  uint64_t spec_pitch, uint32_t nbins,
                                                  uint32_t nblocks,
                                                  const float* __restrict__ scales, HarmArgs a,
                                                  RawPeak* __restrict__ peaks,
-                                                 uint32_t* __restrict__ peak_counts, uint32_t ntiles) {
+                                                 uint32_t* __restrict__ peak_counts, uint32_t ntiles_all,
+                                                 const HarmArgs* __restrict__ jobs,
+                                                 const uint32_t* __restrict__ trial_job) {
     __shared__ double sc[5][HS_T];
     __shared__ double lpos[HS_T];   // k + offset of listed bins
     __shared__ double lsum[HS_T];   // their j = 1 term
@@ -771,6 +777,13 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
     __shared__ uint32_t nlist;
     const uint2 wb = xcd_block();  // neighbouring bins' harmonics share lines on one XCD
     const uint32_t trial = wb.y;
+    const HarmArgs& A = MULTI ? jobs[trial_job[trial]] : a;
+    uint32_t ntiles = ntiles_all;
+    if constexpr (MULTI) {  // this trial's own tile count (uniform: the workgroup leaves together)
+        if (A.p.nlevels == 0 || A.p.end_h[0] <= A.p.first + 1) return;
+        ntiles = (A.p.end_h[0] - A.p.first + 1 + HS_OUT - 1) / HS_OUT;
+        if (wb.x * HS_TILES >= ntiles) return;
+    }
     const size_t row = (size_t)trial * spec_pitch;
     const float* scl = scales + (size_t)trial * nblocks;
     const int tid = threadIdx.x;
@@ -783,11 +796,11 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
         uint32_t r0, r1, r2;
         float s1, s2;
     };
-    auto bin_of = [&](uint32_t tile) { return (int64_t)a.p.first - 1 + (int64_t)tile * HS_OUT + tid; };
+    auto bin_of = [&](uint32_t tile) { return (int64_t)A.p.first - 1 + (int64_t)tile * HS_OUT + tid; };
     auto load_in = [&](uint32_t tile) {
         In v;
         const int64_t k = bin_of(tile);
-        const uint32_t ku = (uint32_t)min(max(k, (int64_t)a.p.first), (int64_t)a.p.end_h[0]);
+        const uint32_t ku = (uint32_t)min(max(k, (int64_t)A.p.first), (int64_t)A.p.end_h[0]);
         if constexpr (F16) {
             // words ku-1 .. ku+2; ku + 2 <= nbins < spec_pitch
             uint64_t q;
@@ -818,20 +831,20 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
         __syncthreads();
         bool listed = false;
         double S1 = 0.0;
-        if (k >= (int64_t)a.p.first && k <= (int64_t)a.p.end_h[0]) {
+        if (k >= (int64_t)A.p.first && k <= (int64_t)A.p.end_h[0]) {
             const uint32_t ku = (uint32_t)k;
             // (raw_power<true> reads the low 16 bits of the word)
             const float p0 = raw_power<F16>(cur.r0), p1 = raw_power<F16>(cur.r1), p2 = raw_power<F16>(cur.r2);
             offset = interp_offset((double)p0, (double)p1, (double)p2);
-            const float pwk = whiten_val(p1, cur.s1, ku, a);
+            const float pwk = whiten_val(p1, cur.s1, ku, A);
             // j = 1 sums the bin at round(k + offset): offset is in [-0.5, 0.5] and
             // round() takes halves away from zero, so that is k or k + 1 (<= nbins - 1),
             // and k + 1 exactly when the rounded sum reaches k + 0.5
             const bool up = (double)ku + offset >= (double)ku + 0.5;
-            const float w1 = up ? whiten_val(p2, cur.s2, ku + 1, a) : pwk;
+            const float w1 = up ? whiten_val(p2, cur.s2, ku + 1, A) : pwk;
             S1 = (double)w1;
             sc[0][tid] = S1 - 1.0;  // / sqrt(1): exact
-            listed = a.p.nlevels > 1 && k <= (int64_t)a.p.end_h[1] && pwk >= 4.0f;
+            listed = A.p.nlevels > 1 && k <= (int64_t)A.p.end_h[1] && pwk >= 4.0f;
         }
         if (listed) {
             const uint32_t slot = atomicAdd(&nlist, 1u);
@@ -850,12 +863,12 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
                 // rounded value gives the same bin)
                 const uint32_t iu = (uint32_t)fmin(round(position), (double)(nbins - 1));
                 const uint32_t w = raw_word<F16>(raw, row + iu);
-                hv[g][jj] = whiten_val(raw_power<F16>(w), scl[iu >> 6], iu, a);
+                hv[g][jj] = whiten_val(raw_power<F16>(w), scl[iu >> 6], iu, A);
             }
             __syncthreads();
             if ((uint32_t)tid < ng) {
                 const uint32_t t = lbin[g0 + tid];
-                const int64_t kk = (int64_t)a.p.first - 1 + (int64_t)tile * HS_OUT + t;
+                const int64_t kk = (int64_t)A.p.first - 1 + (int64_t)tile * HS_OUT + t;
                 float v[HS_NJ];
 #pragma unroll
                 for (int j = 0; j < HS_NJ; j++) v[j] = hv[tid][j];
@@ -867,30 +880,30 @@ __global__ __launch_bounds__(256) void k_harmsum(const void* __restrict__ raw,
                     const int h = 1 << l;
 #pragma unroll
                     for (int j = (h >> 1) + 1; j <= h; j++) S += (double)v[j - 2];
-                    if ((uint32_t)l < a.p.nlevels && kk <= (int64_t)a.p.end_h[l])
+                    if ((uint32_t)l < A.p.nlevels && kk <= (int64_t)A.p.end_h[l])
                         // sqrt(4) and sqrt(16) are powers of two: the product is the exact quotient
-                        sc[l][t] = (l & 1) ? (S - (double)h) / a.sqrt_h[l] : ldexp(S - (double)h, -(l >> 1));
+                        sc[l][t] = (l & 1) ? (S - (double)h) / A.sqrt_h[l] : ldexp(S - (double)h, -(l >> 1));
                 }
             }
             __syncthreads();
         }
-        if (tid >= 1 && tid <= HS_OUT && k >= (int64_t)a.p.first + 1) {
+        if (tid >= 1 && tid <= HS_OUT && k >= (int64_t)A.p.first + 1) {
             double sl[5];
 #pragma unroll
             for (int l = 0; l < 5; l++) sl[l] = sc[l][tid];
 #pragma unroll
             for (int l = 0; l < 5; l++) {
                 const double s = sl[l];
-                if ((uint32_t)l < a.p.nlevels && k <= (int64_t)a.p.end_h[l] - 1 && s >= a.p.threshold &&
+                if ((uint32_t)l < A.p.nlevels && k <= (int64_t)A.p.end_h[l] - 1 && s >= A.p.threshold &&
                     s > sc[l][tid - 1] && s >= sc[l][tid + 1]) {
                     const uint32_t slot = atomicAdd(&peak_counts[trial], 1u);
-                    if (slot < a.p.peak_cap) {
+                    if (slot < A.p.peak_cap) {
                         RawPeak rp;
                         rp.score = s;
                         rp.offset = offset;
                         rp.k = (uint32_t)k;
                         rp.h = 1u << l;
-                        peaks[(size_t)trial * a.p.peak_cap + slot] = rp;
+                        peaks[(size_t)trial * A.p.peak_cap + slot] = rp;
                     }
                 }
             }
@@ -915,11 +928,45 @@ void launch_harmsum(hipStream_t s, const SpecGeom& g, const SearchParams& p, con
     const uint32_t ntiles = (span + HS_OUT - 1) / HS_OUT;
     dim3 grid((ntiles + HS_TILES - 1) / HS_TILES, ntrials);
     if (p.fp16)
-        hipLaunchKernelGGL(k_harmsum<true>, grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch,
-                           g.nbins, g.nblocks, scales, a, peaks, peak_counts, ntiles);
+        hipLaunchKernelGGL((k_harmsum<true, false>), grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch,
+                           g.nbins, g.nblocks, scales, a, peaks, peak_counts, ntiles, nullptr, nullptr);
     else
-        hipLaunchKernelGGL(k_harmsum<false>, grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch,
-                           g.nbins, g.nblocks, scales, a, peaks, peak_counts, ntiles);
+        hipLaunchKernelGGL((k_harmsum<false, false>), grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch,
+                           g.nbins, g.nblocks, scales, a, peaks, peak_counts, ntiles, nullptr, nullptr);
+}
+
+size_t harm_args_bytes() { return sizeof(HarmArgs); }
+
+void fill_harm_args(const SearchParams& p, const uint32_t* birdies, void* out) {
+    HarmArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.p = p;
+    for (int i = 0; i < 2 * MAX_BIRDIE_INTERVALS; i++) a.birdie[i] = i < (int)(2 * p.nbirdie) ? birdies[i] : 0u;
+    static const double SQ[5] = {1.0, 1.4142135623730951, 2.0, 2.8284271247461903, 4.0};
+    for (int l = 0; l < 5; l++) a.sqrt_h[l] = SQ[l];
+    std::memcpy(out, &a, sizeof(a));
+}
+
+void launch_harmsum_multi(hipStream_t s, const SpecGeom& g, int fp16, uint32_t max_tiles, const void* raw_nat,
+                          const float* scales, const void* jobs, const uint32_t* trial_job, uint32_t ntrials,
+                          RawPeak* peaks, uint32_t* peak_counts) {
+    if (max_tiles == 0) return;
+    HarmArgs none;
+    std::memset(&none, 0, sizeof(none));
+    dim3 grid((max_tiles + HS_TILES - 1) / HS_TILES, ntrials);
+    if (fp16)
+        hipLaunchKernelGGL((k_harmsum<true, true>), grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch, g.nbins,
+                           g.nblocks, scales, none, peaks, peak_counts, max_tiles, (const HarmArgs*)jobs,
+                           trial_job);
+    else
+        hipLaunchKernelGGL((k_harmsum<false, true>), grid, dim3(HS_T), 0, s, raw_nat, g.spec_pitch, g.nbins,
+                           g.nblocks, scales, none, peaks, peak_counts, max_tiles, (const HarmArgs*)jobs,
+                           trial_job);
+}
+
+uint32_t harmsum_tiles(const SearchParams& p) {
+    if (p.nlevels == 0 || p.end_h[0] <= p.first + 1) return 0;
+    return (p.end_h[0] - p.first + 1 + HS_OUT - 1) / HS_OUT;
 }
 
 // ------------------------------------------------------------- ranking
@@ -934,13 +981,22 @@ __global__ __launch_bounds__(256) void k_finalize(const RawPeak* __restrict__ pe
                                                   CandidateOut* __restrict__ out,
                                                   uint32_t cap_per_trial,
                                                   uint32_t* __restrict__ counts,
-                                                  uint32_t* __restrict__ overflow) {
+                                                  uint32_t* __restrict__ overflow,
+                                                  const HarmArgs* __restrict__ jobs,
+                                                  const uint32_t* __restrict__ trial_job) {
     __shared__ double s_score[PEAK_CAP];
     __shared__ uint32_t s_hk[PEAK_CAP];
     __shared__ uint16_t s_idx[PEAK_CAP];
     __shared__ double s_kept[256];
     __shared__ uint32_t s_nk;
     const uint32_t trial = blockIdx.x;
+    if (jobs) {  // several observations: this trial's own limits
+        const SearchParams& q = jobs[trial_job[trial]].p;
+        cap = q.peak_cap;
+        bin_hz = q.bin_hz;
+        resolution = q.resolution;
+        max_candidates = q.max_candidates;
+    }
     uint32_t cnt = peak_counts[trial];
     if (cnt > cap) {
         if (threadIdx.x == 0) atomicAdd(overflow, 1u);
@@ -1035,7 +1091,15 @@ void launch_finalize(hipStream_t s, const SpecGeom& g, const SearchParams& p, co
     // the overflow counter lives right after the per-trial counts
     hipLaunchKernelGGL(k_finalize, dim3(ntrials), dim3(256), 0, s, peaks, peak_counts, p.peak_cap,
                        p.bin_hz, p.resolution, p.max_candidates, dms, out, cap_per_trial, counts,
-                       (uint32_t*)(counts + ntrials));
+                       (uint32_t*)(counts + ntrials), (const HarmArgs*)nullptr, (const uint32_t*)nullptr);
+}
+
+void launch_finalize_multi(hipStream_t s, const double* dms, uint32_t ntrials, const RawPeak* peaks,
+                           const uint32_t* peak_counts, const void* jobs, const uint32_t* trial_job,
+                           CandidateOut* out, uint32_t cap_per_trial, uint32_t* counts) {
+    hipLaunchKernelGGL(k_finalize, dim3(ntrials), dim3(256), 0, s, peaks, peak_counts, PEAK_CAP, 0.0, 0.0, 0u,
+                       dms, out, cap_per_trial, counts, (uint32_t*)(counts + ntrials), (const HarmArgs*)jobs,
+                       trial_job);
 }
 
 }  // namespace psr

@@ -128,3 +128,75 @@ def sift(candidates, resolution_hz, limit, reject_dm_zero):
     check(_ffi.lib().psr_sift(arr, n, float(resolution_hz), int(limit), int(bool(reject_dm_zero)), out,
                               C.byref(k)))
     return _cands(out, k.value)
+
+
+class Batch:
+    """psr_batch: the per-trial search of many device-resident observations in
+    one device schedule (one launch per stage per round, a persistent
+    dedispersion kernel over per-XCD work queues, optionally captured in a
+    hipGraph and tuned on the batch's own data).
+
+    observations: Observation list; dms: one DM array per observation;
+    lengths: series length per observation (None/0 = the full available length).
+    run() returns, per observation, its candidate lists in its trial order."""
+
+    def __init__(self, observations, dms, config, lengths=None, cap_per_trial=None, graph=True, autotune=False,
+                 memory_budget=0, ctx=None):
+        self.ctx = ctx or observations[0].ctx
+        self.obs = list(observations)
+        self._dms = [np.ascontiguousarray(d, dtype=np.float64) for d in dms]
+        self.ntrials = [len(d) for d in self._dms]
+        self.cap = int(cap_per_trial or max(1, config.max_candidates))
+        n = len(self.obs)
+        handles = (C.c_void_p * max(1, n))(*[o.handle for o in self.obs])
+        dptr = (C.POINTER(C.c_double) * max(1, n))(*[d.ctypes.data_as(C.POINTER(C.c_double)) for d in self._dms])
+        nt = (C.c_size_t * max(1, n))(*self.ntrials)
+        ln = (C.c_size_t * max(1, n))(*[int(x or 0) for x in (lengths or [0] * n)])
+        opts = _ffi.BatchOpts((_ffi.PSR_BATCH_GRAPH if graph else 0) | (_ffi.PSR_BATCH_AUTOTUNE if autotune else 0), 0,
+                              int(memory_budget))
+        cfg, keep = config._c()
+        self._h = C.c_void_p()
+        check(_ffi.lib().psr_batch_create(self.ctx.handle, handles, n, dptr, nt, ln, C.byref(cfg), self.cap,
+                                          C.byref(opts), C.byref(self._h)))
+        del keep
+        self.rows = sum(self.ntrials)
+        self._out = (_ffi.Candidate * max(1, self.rows * self.cap))()
+        self._counts = np.zeros(max(1, self.rows), dtype=np.uint32)
+
+    def run_raw(self):
+        """Run once; returns (ctypes candidate array, counts) in row order."""
+        check(_ffi.lib().psr_batch_run(self._h, self._out, self._counts.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return self._out, self._counts
+
+    def run(self):
+        out, counts = self.run_raw()
+        res, row = [], 0
+        for k in self.ntrials:
+            lists = []
+            for t in range(k):
+                lists.append([Candidate(out[(row + t) * self.cap + i].frequency_hz,
+                                        out[(row + t) * self.cap + i].period_s, out[(row + t) * self.cap + i].dm,
+                                        int(out[(row + t) * self.cap + i].harmonics),
+                                        out[(row + t) * self.cap + i].score) for i in range(int(counts[row + t]))])
+            res.append(lists)
+            row += k
+        return res
+
+    def info(self):
+        import json
+        n = C.c_size_t()
+        check(_ffi.lib().psr_batch_info(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(_ffi.lib().psr_batch_info(self._h, buf, n.value + 1, C.byref(n)))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if self._h:
+            _ffi.lib().psr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -64,8 +64,12 @@ def _red_noise(seed, nsamples, tsamp, amp):
     return np.fft.irfft(spec * gain, n=nsamples).astype(np.float32)
 
 
-def generate(w, threads=0):
-    """Packed payload (uint8 array) of workload `w`."""
+def generate(w, threads=0, device=False):
+    """Packed payload (uint8 array) of workload `w`.
+
+    device=True generates it on the GPU (csrc/k_synth.hip: the same model,
+    deterministic in the seed, much faster for large batches; 1/2/4/8-bit
+    samples in whole-byte frames only) and copies it back."""
     ps = (_ffi.SynthPulsar * max(1, len(w.pulsars)))(*[_ffi.SynthPulsar(*p) for p in w.pulsars])
     b = np.ascontiguousarray(np.array(w.bursts, dtype=np.float64).reshape(-1))
     rc_ = np.ascontiguousarray(np.array(w.rfi_chans, dtype=np.uint32))
@@ -77,8 +81,11 @@ def generate(w, threads=0):
         w.sine[0], w.sine[1],
         rc_.ctypes.data_as(C.POINTER(C.c_uint32)) if rc_.size else None, len(w.rfi_chans), w.rfi_amp,
         common.ctypes.data_as(C.POINTER(C.c_float)) if common is not None else None, threads)
-    out = np.zeros(w.payload_bytes, dtype=np.uint8)
-    rc = _ffi.synth_lib().psr_synth_generate(C.byref(cfg), out.ctypes.data, out.nbytes)
+    out = np.empty(w.payload_bytes, dtype=np.uint8) if device else np.zeros(w.payload_bytes, dtype=np.uint8)
+    if device:
+        rc = _ffi.synth_gpu_lib().psr_synth_generate_gpu(C.byref(cfg), out.ctypes.data, out.nbytes)
+    else:
+        rc = _ffi.synth_lib().psr_synth_generate(C.byref(cfg), out.ctypes.data, out.nbytes)
     if rc != 0:
         raise RuntimeError("psr_synth_generate failed (%d)" % rc)
     return out
@@ -138,3 +145,48 @@ def workload(name, nsamples=None):
                         pulsars=pulsars, bursts=bursts, rfi_chans=rfi, rfi_amp=3.0 if rfi else 0.0,
                         red_noise=red, dms=dms, max_harmonics=16, top=20 if headline else 5)
     raise KeyError(name)
+
+
+def batch_workloads(count=256, seed=4096, log2_len=(16, 21), trials=(50, 2000), controls=0.2):
+    """The batching workload: `count` seeded observations of uneven size.
+
+    Observation i (seed + i) draws its channel count from {64, 96, 256, 512,
+    1024}, its bit depth from {2, 4, 8}, tsamp uniform in 50-100 us, its
+    length as 2^U(log2_len), and its DM plan as 50-2000 trials from DM 0 at the
+    reference's default step (dedisperse::dm_step of its band).  A fraction
+    `controls` of the observations are noise/RFI-only controls; the others
+    carry one or two pulsars (period log-uniform 1-500 ms, duty 1-10 %, SNR
+    6-30, DM within the plan).  Bands: 64 channels over 1350-1450 MHz, 96
+    over 1352-1448 MHz, 256-1024 over 1200-1600 MHz (descending channels)."""
+    from .dedisperse import dm_step
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng(seed + i)
+        nch = int(rng.choice([64, 96, 256, 512, 1024]))
+        nbits = int(rng.choice([2, 4, 8]))
+        tsamp = float(rng.uniform(50e-6, 100e-6))
+        ns = int(round(2.0 ** rng.uniform(*log2_len)))
+        lo, hi = {64: (1350.0, 1450.0), 96: (1352.0, 1448.0)}.get(nch, (1200.0, 1600.0))
+        foff = -(hi - lo) / nch
+        fch1 = hi + foff / 2
+        mean, sigma = {2: (1.5, 0.8), 4: (8.0, 1.5), 8: (128.0, 16.0)}[nbits]
+        freqs = fch1 + np.arange(nch) * foff
+        step = dm_step(freqs, tsamp)
+        ntr = int(rng.integers(trials[0], trials[1] + 1))
+        # keep the largest delay (about ntr samples at the default step) below a quarter of the length
+        ntr = max(trials[0], min(ntr, ns // 4))
+        dms = np.arange(ntr, dtype=np.float64) * step
+        pulsars = []
+        control = rng.uniform() < controls
+        if not control:
+            for _ in range(int(rng.integers(1, 3))):
+                P = float(np.exp(rng.uniform(np.log(1e-3), np.log(0.5))))
+                duty = float(rng.uniform(0.01, 0.10))
+                snr = float(rng.uniform(6, 30))
+                dm = float(rng.uniform(0.05, 0.9) * dms[-1])
+                pulsars.append((P, dm, duty, amplitude_for_snr(snr, nch, ns, duty), float(rng.uniform(0, 1))))
+        rfi = sorted(rng.choice(nch, max(1, nch // 20), replace=False).tolist()) if rng.uniform() < 0.3 else []
+        out.append(Workload("batch-%03d%s" % (i, "-control" if control else ""), seed + i, ns, nch, nbits, tsamp,
+                            fch1, foff, mean, sigma, pulsars=pulsars, rfi_chans=rfi, rfi_amp=2.0 if rfi else 0.0,
+                            dms=dms, min_hz=10.0, max_hz=2000.0, threshold=8.0, max_harmonics=16, top=5))
+    return out

@@ -124,7 +124,8 @@ def check_large(name):
     print("fp16 vs f64 oracle: %s" % json.dumps(m), flush=True)
     assert m["total"] >= len(sel) and m["matched"] >= 0.9 * m["total"]
     assert m["top_matched"] >= 0.9 * m["trials"]
-    assert m["max_rel_snr_error"] <= 1e-2
+    assert m["max_rel_snr_error_sharp"] <= 1e-2 and m["max_rel_snr_error"] <= 5e-2
+    assert m["flat_peaks"] <= max(2, 0.01 * m["matched"])
     # FP16 spectral power against the f64 spectrum on four trials (the strongest leaders' and two spaced)
     lead_trials = [int(np.nonzero(w.dms == l["dm"])[0][0]) for l in leaders32[:2]]
     worst_norm = 0.0
