@@ -88,39 +88,149 @@ def traffic_per_unit():
     return MEASURED_TRAFFIC_BYTES_PER_UNIT
 
 
-def fp64_error(obs, w, n, out16, c16, cap, ntrials):
-    """Candidate-SNR error of the FP16 search against the FP64 oracle on the
-    trials holding the strongest FP16 candidates (full-length series)."""
+def host_cores():
+    """(threads to use, cores visible, CPU model): the box's CPU share is 16
+    cores per GPU whatever the affinity mask shows."""
+    visible = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return min(16, visible), visible, model
+
+
+def cpu_baseline(payload, w, n, ntrials):
+    """The oracle's per-trial loop (restated reference algorithm) on a sample of
+    trials, on the host cores: one warm-up run, then the median of three
+    (the reference's own protocol, scripts/benchmark.py:22-31)."""
     import oracle
-    from pulsarsr_amd import dedisperse
+    threads, visible, model = host_cores()
+    data, _ = oracle.decode(payload, w.nbits, w.nchans)
+    sel = w.dms[np.linspace(0, len(w.dms) - 1, ntrials).astype(int)]
+    kw = dict(min_hz=w.min_hz, max_hz=w.max_hz, threshold=w.threshold, max_candidates=32,
+              max_harmonics=w.max_harmonics)
+    oracle.search_trials(data, w.nsamples, w.frequencies(), w.tsamp, sel[:threads], n, threads=threads, **kw)
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.search_trials(data, w.nsamples, w.frequencies(), w.tsamp, sel, n, threads=threads, **kw)
+        runs.append(time.perf_counter() - t0)
+        log("cpu baseline run: %.2f s" % runs[-1])
+    dt = float(np.median(runs))
+    return dict(value=ntrials * n / dt, unit="DM-trial*samples/s", cores=threads, kind="port",
+                host_cores_visible=visible, cpu=model, runs_s=runs,
+                sample="%d of %d DM trials (evenly spaced), full %d-sample series each; one warm-up, median of "
+                       "3 runs (%.2f s)" % (ntrials, len(w.dms), n, dt))
+
+
+def fp64_report(obs, w, n, out16, c16, cap, ntrials):
+    """FP16 storage against the all-f64 oracle on `ntrials` trials (the ones
+    holding the strongest FP16 candidates): candidate-SNR error over matched
+    candidates and the spectral-power error (normwise and worst bin relative to
+    its whitening level) of the stored FP16 spectrum."""
+    import oracle
+    from pulsarsr_amd import PRECISION_FP16, dedisperse, search
     best = sorted(range(len(c16)), key=lambda t: -max([out16[t * cap + i].score for i in range(int(c16[t]))] or [0]))
-    worst, matched = 0.0, 0
+    worst, matched, total, pnorm, pbin = 0.0, 0, 0, 0.0, 0.0
     for t in best[:ntrials]:
         x = dedisperse.apply_dms(obs, [float(w.dms[t])], n)[0]
         ref = oracle.periodicity(x, w.tsamp, float(w.dms[t]), f64=True, min_hz=w.min_hz, max_hz=w.max_hz,
                                  threshold=w.threshold, max_candidates=32, max_harmonics=w.max_harmonics)
+        nfft = 2 * (1 << (n - 1).bit_length())
+        bin_hz = 1.0 / (nfft * w.tsamp)
         for i in range(int(c16[t])):
             g = out16[t * cap + i]
-            hit = [r for r in ref if r["harmonics"] == g.harmonics and abs(r["frequency_hz"] - g.frequency_hz) < 1e-6 * g.frequency_hz]
+            total += 1
+            hit = [r for r in ref if r["harmonics"] == g.harmonics and abs(r["frequency_hz"] - g.frequency_hz) <= 0.05 * bin_hz]
             if hit:
                 worst = max(worst, abs(g.score - hit[0]["score"]) / abs(hit[0]["score"]))
                 matched += 1
-    return dict(max_rel_snr_error=worst, matched=matched, trials=ntrials)
+        raw16, _ = search.spectrum(x, w.tsamp, search.Config(min_hz=w.min_hz, precision=PRECISION_FP16))
+        raw64, white64 = oracle.spectrum_f64(x, w.tsamp, min_hz=w.min_hz)
+        p16 = raw16.astype(np.float64) * n  # stored amplitudes are |X| / sqrt(n)
+        pnorm = max(pnorm, float(np.linalg.norm(p16 - raw64) / np.linalg.norm(raw64)))
+        ok = white64 > 0  # the whitening scale raw / white, where defined (not DC or birdies)
+        level = raw64[ok] / white64[ok]
+        # each bin's error relative to the larger of its power and its whitening level
+        pbin = max(pbin, float(np.max(np.abs(p16 - raw64)[ok] / np.maximum(raw64[ok], level))))
+    return dict(max_rel_snr_error=worst, matched=matched, candidates=total, trials=ntrials,
+                max_spectral_power_error_normwise=pnorm, max_spectral_power_error_per_bin=pbin)
 
 
-def cpu_baseline(payload, w, n, ntrials, threads):
-    """The oracle's per-trial loop (restated reference algorithm) on a sample of trials."""
-    import oracle
-    data = np.ascontiguousarray(payload.reshape(w.nsamples, w.nchans).T, dtype=np.float32)
-    sel = w.dms[np.linspace(0, len(w.dms) - 1, ntrials).astype(int)]
+def measure_config(name, steps, fp64_trials):
+    """One smaller workload: FP16 and FP32 rates and the FP16 errors against FP64."""
+    from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, synth
+    w = synth.workload(name)
+    payload = synth.generate(w, threads=host_cores()[0])
+    n = discovery_length(w)
+    obs = dedisperse.Observation.from_packed(payload, w.nbits, w.nchans, w.tsamp, w.fch1, w.foff)
+    res = {"nchans": w.nchans, "nbits": w.nbits, "nsamples": w.nsamples, "trials": len(w.dms), "series_len": n}
+    units = len(w.dms) * n
+    for prec, key in ((PRECISION_FP16, "fp16"), (PRECISION_FP32, "fp32")):
+        run_search(obs, w, n, prec)
+        ts = []
+        for _ in range(steps):
+            dt, out, cnt, cap, st = run_search(obs, w, n, prec)
+            ts.append(dt)
+        ms = 1e3 * float(np.median(ts))
+        res[key] = {"value": units / (ms * 1e-3), "ms_per_step": ms, "dedisperse_ms": st["dedisperse_ms"],
+                    "spectral_ms": st["fft_ms"] + st["whiten_ms"] + st["harmsum_ms"]}
+        if key == "fp16":
+            out16, c16, cap16 = out, cnt.copy(), cap
+    res["fp16_over_fp32"] = res["fp32"]["ms_per_step"] / res["fp16"]["ms_per_step"]
+    if fp64_trials:
+        res["fp64_check"] = fp64_report(obs, w, n, out16, c16, cap16, fp64_trials)
+    obs.close()
+    log("%s: fp16 %.1f ms, fp32 %.1f ms" % (name, res["fp16"]["ms_per_step"], res["fp32"]["ms_per_step"]))
+    return res
+
+
+def measure_batching(steps):
+    """The batching form: 256 observations of uneven size (channels 64-1024,
+    2/4/8-bit, 2^16-2^21 samples, 50-2000 trials), generated on the GPU, made
+    resident, then searched as one autotuned batch captured in a hipGraph."""
+    from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, search, synth
+    ws = synth.batch_workloads()
     t0 = time.perf_counter()
-    oracle.search_trials(data, w.nsamples, w.frequencies(), w.tsamp, sel, n, threads=threads,
-                         min_hz=w.min_hz, max_hz=w.max_hz, threshold=w.threshold, max_candidates=32,
-                         max_harmonics=w.max_harmonics)
-    dt = time.perf_counter() - t0
-    return dict(value=ntrials * n / dt, unit="DM-trial*samples/s", cores=threads, kind="port",
-                sample="%d of %d DM trials (evenly spaced), full %d-sample series each, %.1f s"
-                       % (ntrials, len(w.dms), n, dt))
+    obs = []
+    for w in ws:
+        p = synth.generate(w, device=True)
+        obs.append(dedisperse.Observation.from_packed(p, w.nbits, w.nchans, w.tsamp, w.fch1, w.foff))
+        del p
+    lengths = [discovery_length(w) for w in ws]
+    units = sum(len(w.dms) * n for w, n in zip(ws, lengths))
+    log("batching workload resident: %d observations, %d trials, %.1f s" % (len(ws), sum(len(w.dms) for w in ws),
+                                                                          time.perf_counter() - t0))
+    res = {"observations": len(ws), "trials": sum(len(w.dms) for w in ws), "units": units}
+    for prec, key in ((PRECISION_FP16, "fp16"), (PRECISION_FP32, "fp32")):
+        cfg = search.Config(min_hz=10.0, max_hz=2000.0, threshold=8.0, max_candidates=32, max_harmonics=16,
+                            precision=prec)
+        t0 = time.perf_counter()
+        b = search.Batch(obs, [w.dms for w in ws], cfg, lengths=lengths, graph=True, autotune=True)
+        setup = time.perf_counter() - t0
+        b.run_raw()
+        ts = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            b.run_raw()
+            ts.append(time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(ts))
+        info = b.info()
+        res[key] = {"value": units / (ms * 1e-3), "ms_per_step": ms, "device_ms": info["last_ms"],
+                    "setup_s": setup, "rounds": info["rounds"]}
+        if key == "fp16":
+            res["autotune"] = {"fft": info["fft"], "dedisperse_span": info["dedisperse_span"],
+                               "dedisperse_wg_per_cu": info["dedisperse_wg_per_cu"]}
+        b.close()
+        log("batching %s: %.1f ms per step (%d rounds)" % (key, ms, info["rounds"]))
+    res["fp16_over_fp32"] = res["fp32"]["ms_per_step"] / res["fp16"]["ms_per_step"]
+    for o in obs:
+        o.close()
+    return res
 
 
 def main():
@@ -129,7 +239,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="headline-1024ch-8bit")
-    ap.add_argument("--cpu-trials", type=int, default=160, help="trials in the CPU baseline sample (~10 s)")
+    ap.add_argument("--cpu-trials", type=int, default=96, help="trials in the CPU baseline sample (~5 s per run)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the smaller workloads and the batching form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32-steps", type=int, default=1)
     ap.add_argument("--trials", type=int, default=0, help="use the first N DM trials only (profiling runs)")
@@ -156,7 +267,7 @@ def main():
     if a.trials:
         w.dms = w.dms[:a.trials]
     t0 = time.perf_counter()
-    payload = synth.generate(w, threads=min(16, os.cpu_count() or 1))
+    payload = synth.generate(w, threads=host_cores()[0])
     log("generated %.2f GB in %.1f s" % (payload.nbytes / 1e9, time.perf_counter() - t0))
     n = discovery_length(w)
     t0 = time.perf_counter()
@@ -223,7 +334,8 @@ def main():
         "unit": "DM-trial*samples/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms16,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u8", "data": "synthetic (seeded generator, pulsars+RFI, generated in-process)",
+        "dtype": "fp16-storage/fp32-compute (u8 samples, exact integer channel sums)",
+        "data": "synthetic (seeded generator, pulsars+RFI, generated in-process)",
         "config": {"workload": w.name, "nchans": w.nchans, "nbits": w.nbits, "nsamples": w.nsamples,
                    "trials": len(w.dms), "series_len": n, "fft_len": int(stages["fft_len"]),
                    "harmonics": w.max_harmonics, "storage": "fp16"},
@@ -249,9 +361,14 @@ def main():
         "launches_per_step": launches,
     }
     if not a.no_cpu:
-        res["fp64_check"] = fp64_error(obs, w, n, out16, c16, cap, a.fp64_trials)
+        res["fp64_check"] = fp64_report(obs, w, n, out16, c16, cap, a.fp64_trials)
+    obs.close()
+    if not a.no_configs:
+        res["configs"] = {name: measure_config(name, 3, a.fp64_trials if not a.no_cpu else 0)
+                          for name in ("plain-64ch-8bit", "lband-96ch-4bit", "precision-1024ch-8bit")}
+        res["configs"]["batching-256obs"] = measure_batching(3)
     if not a.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(payload, w, n, a.cpu_trials, min(16, os.cpu_count() or 1))
+        res["cpu_baseline"] = cpu_baseline(payload, w, n, a.cpu_trials)
     print(json.dumps(res), flush=True)
 
 
