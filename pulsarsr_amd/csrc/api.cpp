@@ -822,8 +822,14 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
     const size_t per_trial = (serial ? 1 : 2) * (size_t)g.series_pitch * es +
                              3 * (size_t)g.spec_pitch * es + (size_t)PEAK_CAP * sizeof(RawPeak) +
                              cap_per_trial * sizeof(CandidateOut) + (size_t)o->nchans * 4 + 4096;
-    // (half the ctx memory cap when one is set); whole 64-trial dedispersion blocks
+    // (half the free device memory, and half the ctx memory cap when one is set);
+    // whole 64-trial dedispersion blocks
     size_t budget = (size_t)64 << 30;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b) {
+        // the ctx's own buffers are reused, so they count as free for this call
+        budget = std::min<size_t>(budget, (free_b + (size_t)c->mem_used) / 2);
+    }
     if (c->mem_cap) budget = std::min<size_t>(budget, c->mem_cap / 2);
     size_t chunk = std::max<size_t>(1, budget / per_trial);
     if (chunk >= DEDISP_BLOCK_TRIALS) chunk -= chunk % DEDISP_BLOCK_TRIALS;
