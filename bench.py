@@ -72,20 +72,26 @@ def top_families(cands, n, tsamp, k=20):
     return leaders
 
 
-# HBM bytes per unit of each kernel on the FP16 path (FETCH_SIZE + WRITE_SIZE,
-# calibrated against a 1 GiB device copy), from the rocprofv3 passes recorded in
-# profiles/hbm_traffic_headline-1024ch-8bit.md (one 768-trial launch).
-MEASURED_TRAFFIC_BYTES_PER_UNIT = {
-    "k_dedisp_u8": 7.53,
-    "k_fft_pass1r": 10.03,
-    "k_fft_pass2r": 10.04,
-    "k_whiten_fused16": 5.89,
-    "k_harmsum": 1.82,
+# HBM bytes per launch of each kernel on the FP16 path of the headline step (all
+# 4000 trials in one dedispersion launch; FETCH_SIZE + WRITE_SIZE calibrated
+# against a 1 GiB device copy), from the rocprofv3 passes over this script's own
+# launch recorded in profiles/hbm_traffic_headline-1024ch-8bit.md.  They describe
+# that launch shape only: other workloads or trial counts report traffic null.
+MEASURED_LAUNCH = {"workload": "headline-1024ch-8bit", "trials": 4000, "launches": 1}
+MEASURED_TRAFFIC_GB_PER_LAUNCH = {
+    "k_dedisp_u8": 91.40 + 17.14,
+    "k_fft_pass1r": 16.78 + 67.11,
+    "k_fft_pass2r": 67.25 + 16.81,
+    "k_whiten_fused16": 31.77 + 17.35,
+    "k_harmsum": 15.22 + 0.02,
 }
 
 
-def traffic_per_unit():
-    return MEASURED_TRAFFIC_BYTES_PER_UNIT
+def traffic_gb(kernel, w, launches):
+    if (w.name, len(w.dms), launches) != (MEASURED_LAUNCH["workload"], MEASURED_LAUNCH["trials"],
+                                        MEASURED_LAUNCH["launches"]):
+        return None
+    return MEASURED_TRAFFIC_GB_PER_LAUNCH.get(kernel)
 
 
 def host_cores():
@@ -233,6 +239,25 @@ def measure_batching(steps):
     return res
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime: a
+    transfer of known size for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(mib << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, nbytes)
+    if hip.hipMemcpy(dst, src, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log("calibration copy: %d MiB" % mib)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,13 +281,7 @@ def main():
 
     from pulsarsr_amd import PRECISION_FP16, PRECISION_FP32, dedisperse, synth
     if a.copy_calibration_mib:
-        import torch
-        src = torch.ones(a.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("calibration copy: %d MiB" % a.copy_calibration_mib)
-        del src, dst
+        copy_calibration(a.copy_calibration_mib)
     w = synth.workload(a.workload)
     if a.trials:
         w.dms = w.dms[:a.trials]
@@ -318,16 +337,15 @@ def main():
     adds = float(units) * w.nchans
     ach_tadds = adds / (dd_ms * 1e-3) / 1e12
     ach_lds = adds * LDS_BYTES_PER_ADD / (dd_ms * 1e-3) / 1e9
-    tpu = traffic_per_unit()
     launches = int(stages.get("launches", 0)) or 1
-    units_per_launch = float(units) / launches
-    dd_traffic = tpu.get("k_dedisp_u8")
+    dd_traffic = traffic_gb("k_dedisp_u8", w, launches)
     # spectral chain bytes (FP16 storage): series read 2, spectrum write 2r, whitening
     # read 2r + write 2r, harmonic-sum read 2r per unit (r = bins per series sample)
     r = (stages["fft_len"] // 2 + 1) / n
     spec_ms = stages["fft_ms"] + stages["whiten_ms"] + stages["harmsum_ms"]
     spec_alg = 2 + 2 * r + 2 * r + 2 * r
-    spec_traffic = sum(tpu.get(k, 0.0) for k in ("k_fft_pass1r", "k_fft_pass2r", "k_whiten_fused16", "k_harmsum")) or None
+    spec_parts = [traffic_gb(k, w, launches) for k in ("k_fft_pass1r", "k_fft_pass2r", "k_whiten_fused16", "k_harmsum")]
+    spec_traffic = sum(spec_parts) if all(x is not None for x in spec_parts) else None
     res = {
         "metric": "DM-trial·samples/s (dedisp+FFT+harmsum), FP16-store vs FP32 at same candidates",
         "value": units / (ms16 * 1e-3),
@@ -348,16 +366,22 @@ def main():
         "ingest_s": ingest_s,
         "roofline": {"bound": "lds", "kernel": "k_dedisp_u8", "achieved": ach_lds, "peak": LDS_PEAK_GBS,
                      "unit": "GB/s", "frac": ach_lds / LDS_PEAK_GBS,
-                     "traffic": dd_traffic * units_per_launch if dd_traffic else None,
+                     "traffic": dd_traffic * 1e9 if dd_traffic else None,
                      "valu": {"achieved": ach_tadds, "peak": VALU_PEAK_TADDS, "unit": "Tadd/s",
                               "frac": ach_tadds / VALU_PEAK_TADDS},
                      "note": "1 LDS byte per add (channel-pair rows), adds = trials x series_len x nchans; "
-                             "HIP-event time of the dedispersion stage; traffic = HBM bytes per launch"},
+                             "HIP-event time of the dedispersion stage; traffic = HBM bytes of this launch "
+                             "(FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_headline-1024ch-8bit.md): the "
+                             "filterbank is re-read from the fabric about 21 times per launch, 11.0 of 13.1 B/unit",
+                     "hbm": {"achieved": (units * 2.0 + w.nchans * w.nsamples) / (dd_ms * 1e-3) / 1e9,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": (units * 2.0 + w.nchans * w.nsamples) / (dd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "note": "algorithmic bytes: FP16 series write + the filterbank read once"}},
         "roofline_spectral": {"bound": "hbm", "kernels": "fft+whiten+harmsum",
                               "achieved": float(units) * spec_alg / (spec_ms * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": float(units) * spec_alg / (spec_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                              "traffic": spec_traffic * units_per_launch if spec_traffic else None},
+                              "traffic": spec_traffic * 1e9 if spec_traffic else None},
         "launches_per_step": launches,
     }
     if not a.no_cpu:

@@ -814,9 +814,9 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
     std::vector<double> sorted(ntrials);
     for (size_t i = 0; i < ntrials; i++) sorted[i] = dms[order[i]];
     // trials per chunk: keep the working set (the series slots, one set of
-    // spectral buffers) near 64 GiB of the 288 GiB HBM -- fewer, longer launches
-    // (3840 + 160 trials at 1024 x 2^22 in FP16) measured 2 ms per step faster
-    // than 16 or 32 GiB chunks
+    // spectral buffers) within 128 GiB of the 288 GiB HBM -- fewer, longer
+    // launches measured faster than 16 or 32 GiB chunks, and 4000 trials of
+    // 2^22 samples then run as one launch in FP16 and in FP32
     const bool serial = (c->flags & PSR_CTX_OVERLAP) == 0;
     const size_t es = p.fp16 ? 2 : 4;
     const size_t per_trial = (serial ? 1 : 2) * (size_t)g.series_pitch * es +
@@ -824,7 +824,7 @@ int psr_search(psr_ctx* c, psr_obs* o, const double* dms, size_t ntrials, size_t
                              cap_per_trial * sizeof(CandidateOut) + (size_t)o->nchans * 4 + 4096;
     // (half the free device memory, and half the ctx memory cap when one is set);
     // whole 64-trial dedispersion blocks
-    size_t budget = (size_t)64 << 30;
+    size_t budget = (size_t)128 << 30;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b) {
         // the ctx's own buffers are reused, so they count as free for this call
