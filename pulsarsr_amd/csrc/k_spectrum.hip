@@ -236,8 +236,22 @@ __global__ __launch_bounds__(256) void k_sort_blocks(
     for (int c = 0; c < 8; c++) store_keys8<F16>(sorted, o + 8 * c, v + 8 * c);
 }
 
+// Staged sorted blocks: block lb starts at element lb * STRIDE + one extra word
+// per 8 blocks.  The stride alone (an odd number of words) spreads consecutive
+// blocks over the banks, as the window medians read them; the extra word also
+// spreads blocks 8 apart, which the lanes of a wave store together (lane l holds
+// block ch + 8 col for 512-row columns, 16 or 32 apart for longer ones).
+template <typename KeyT, int STRIDE>
+__device__ __forceinline__ uint32_t key_block(uint32_t lb) {
+    return lb * (uint32_t)STRIDE + (lb >> 3) * (uint32_t)(4 / sizeof(KeyT));
+}
+template <typename KeyT, int STRIDE>
+constexpr uint32_t key_blocks_elems(uint32_t nb) {  // elements of nb staged blocks
+    return nb * (uint32_t)STRIDE + ((nb + 7) >> 3) * (uint32_t)(4 / sizeof(KeyT));
+}
+
 // Whitening scale of block b (lib.rs:45-60) from the sorted blocks b-2 .. b+2
-// at s + (lb + j) * STRIDE: the exact window median is found by a cut search --
+// at s + key_block(lb + j): the exact window median is found by a cut search --
 // a position p_j in every block with sum p_j = len/2 such that all keys left of
 // the cut are <= all keys right of it; the median is then the smallest key
 // right of the cut.  Blocks outside [0, nblocks) hold nothing and are not read.
@@ -259,7 +273,7 @@ __device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t n
             cnt = s1 > s0 ? (int)(s1 - s0) : 0;
         }
         nn[j] = cnt;
-        A[j] = s + (size_t)(lb + j) * STRIDE;
+        A[j] = s + key_block<KeyT, STRIDE>(lb + j);
         p[j] = cnt >> 1;
         len += cnt;
     }
@@ -322,8 +336,8 @@ __device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t n
             nu = j == ju ? nn[j] : nu;
         }
         // jl != ju (within one sorted block L <= U)
-        const KeyT* Al = s + (lb + jl) * STRIDE;
-        const KeyT* Au = s + (lb + ju) * STRIDE;
+        const KeyT* Al = s + key_block<KeyT, STRIDE>(lb + jl);
+        const KeyT* Au = s + key_block<KeyT, STRIDE>(lb + ju);
         auto below = [](int i) { return i < 0 ? 0 : i; };
         auto above = [](int i) { return i > 63 ? 63 : i; };
         const uint32_t a2 = Al[below(pl - 2)], a3 = Al[below(pl - 3)], a4 = Al[below(pl - 4)],
@@ -372,7 +386,7 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
     // which halves the footprint and doubles the workgroups per CU
     using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
     constexpr int STRIDE = F16 ? 2 * 33 : (int)MB_STRIDE;
-    __shared__ KeyT s[(MB_BLOCKS + 4) * STRIDE];
+    __shared__ KeyT s[key_blocks_elems<KeyT, STRIDE>(MB_BLOCKS + 4)];
     constexpr int E = F16 ? 8 : 4;                                  // keys per 16-byte load
     constexpr int NV = (int)((MB_BLOCKS + 4) * 64 / E);             // vectors staged
     constexpr int VPT = (NV + (int)MB_BLOCKS - 1) / (int)MB_BLOCKS;  // per thread
@@ -407,7 +421,7 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
             // block `nsorted` (m a multiple of 64) holds Nyquist alone
             if (gb == (int64_t)nsorted && gb < (int64_t)nblocks && i0 + e == 0)
                 key = load_key(spec_t, F16, srow + m);
-            s[lb * STRIDE + i0 + e] = (KeyT)key;  // FP16: KEY_INF keeps its low 16 bits
+            s[key_block<KeyT, STRIDE>(lb) + i0 + e] = (KeyT)key;  // FP16: KEY_INF keeps its low 16 bits
         }
     }
     __syncthreads();
@@ -435,7 +449,7 @@ constexpr int wf_stride() { return F16 ? 2 * 33 : (int)MB_STRIDE; }
 template <bool F16>
 constexpr size_t wf_lds_bytes() {
     using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
-    return (size_t)(SB_BLOCKS + 6) * wf_stride<F16>() * sizeof(KeyT) + 4 * 64 * 4;  // + side blocks
+    return (size_t)key_blocks_elems<KeyT, wf_stride<F16>()>(SB_BLOCKS + 6) * sizeof(KeyT) + 4 * 64 * 4;  // + side blocks
 }
 template <bool F16>
 __global__ __launch_bounds__(256) void k_whiten_fused(
@@ -504,14 +518,14 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
         sort64_packed(w);
 #pragma unroll
         for (int i = 0; i < 32; i++) {
-            sk[(B + 2) * STRIDE + i] = w[i].x;
-            sk[(B + 2) * STRIDE + i + 32] = w[i].y;
+            sk[key_block<KeyT, STRIDE>(B + 2) + i] = w[i].x;
+            sk[key_block<KeyT, STRIDE>(B + 2) + i + 32] = w[i].y;
         }
     } else {
         if (k_lo == 0 && B == 0) v[0] = KEY_INF;  // DC is not part of any window
         bitonic_sort_regs<64>(v);
 #pragma unroll
-        for (int i = 0; i < 64; i++) sk[(B + 2) * STRIDE + i] = (KeyT)v[i];
+        for (int i = 0; i < 64; i++) sk[key_block<KeyT, STRIDE>(B + 2) + i] = (KeyT)v[i];
     }
     __syncthreads();  // side keys and the floor are in LDS
     // side block w (waves 0..3): bitonic network across the wave's lanes
@@ -525,9 +539,9 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
             const bool up = (lane & k) == 0, lower = (lane & j) == 0;
             x = lower == up ? min(x, pv) : max(x, pv);
         }
-    sk[(wave < 2 ? wave : SB_BLOCKS + wave) * STRIDE + lane] = (KeyT)x;
-    if (threadIdx.x < 2 * STRIDE)  // local blocks SB + 4 and SB + 5: never counted, kept defined
-        sk[(SB_BLOCKS + 4) * STRIDE + threadIdx.x] = (KeyT)KEY_INF;
+    sk[key_block<KeyT, STRIDE>(wave < 2 ? wave : SB_BLOCKS + wave) + lane] = (KeyT)x;
+    if (threadIdx.x < 2 * STRIDE)  // local blocks SB + 4 and SB + 5 (contiguous): never counted, kept defined
+        sk[key_block<KeyT, STRIDE>(SB_BLOCKS + 4) + threadIdx.x] = (KeyT)KEY_INF;
     __syncthreads();
     const float fl = sfloor;
     const uint32_t bm = threadIdx.x;  // medians in natural block order
@@ -618,8 +632,8 @@ __global__ __launch_bounds__(256) void k_whiten_fused16(
         sort64_packed(v);
 #pragma unroll
         for (int i = 0; i < 32; i++) {
-            sk[(B + 2) * STRIDE + i] = v[i].x;
-            sk[(B + 2) * STRIDE + i + 32] = v[i].y;
+            sk[key_block<uint16_t, STRIDE>(B + 2) + i] = v[i].x;
+            sk[key_block<uint16_t, STRIDE>(B + 2) + i + 32] = v[i].y;
         }
         __syncthreads();  // side keys and the floor are in LDS
         // side block w (waves 0..3): bitonic network across the wave's lanes
@@ -632,9 +646,9 @@ __global__ __launch_bounds__(256) void k_whiten_fused16(
                 const bool up = (lane & k) == 0, lower = (lane & j) == 0;
                 x = lower == up ? min(x, pv) : max(x, pv);
             }
-        sk[(wave < 2 ? wave : SB_BLOCKS + wave) * STRIDE + lane] = (uint16_t)x;
-        if (threadIdx.x < 2 * STRIDE)  // local blocks SB + 4 and SB + 5: never counted, kept defined
-            sk[(SB_BLOCKS + 4) * STRIDE + threadIdx.x] = (uint16_t)KEY_INF;
+        sk[key_block<uint16_t, STRIDE>(wave < 2 ? wave : SB_BLOCKS + wave) + lane] = (uint16_t)x;
+        if (threadIdx.x < 2 * STRIDE)  // local blocks SB + 4 and SB + 5 (contiguous): never counted, kept defined
+            sk[key_block<uint16_t, STRIDE>(SB_BLOCKS + 4) + threadIdx.x] = (uint16_t)KEY_INF;
         __syncthreads();
         const float fl = sfloor;
         const uint32_t bm = threadIdx.x;  // medians in natural block order
