@@ -139,10 +139,10 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 }
 
 // ------------------------------------------------- integer dedispersion (u8)
-// Workgroup = 8 waves sharing one time tile of DD_TILE = 512 outputs and a block
-// of DD_TRIALS = 64 consecutive (ascending) DM trials; each wave owns DD_M = 8
-// trials, and lane l owns outputs t0 + l + 64 k (k < 8).  A channel's samples
-// are staged into LDS as 16-byte words of eight u16 lanes,
+// Workgroup = 8 waves sharing one time tile of DD_TILE = 512 outputs and a
+// block of DD_TRIALS = 64 consecutive (ascending) DM trials; each wave owns
+// DD_M = 8 trials, and lane l owns outputs t0 + l + 64 k (k < 8).  A channel's
+// samples are staged into LDS as 16-byte words of eight u16 lanes,
 //     w[i] = (s[base + i + 64 k], k < 8)      (dword j: k = 2j low, 2j + 1 high),
 // with base = the block's smallest delay of the channel rounded down to 4.  A
 // trial whose delay is base + e reads word e + l in lane l with one aligned
@@ -165,15 +165,19 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // sums mod 2^16 whose carries are counted in 4-bit fields, so totals up to 2^20
 // are exact.
 //
-// A planner kernel lays out each block as stages: consecutive slots (a slot is
-// a channel pair or a single channel) whose rows fit the LDS row area and whose
-// raw windows (one per channel) fit the raw area.  A stage is one fixed-size
-// record -- header, row records and every trial's 16-bit LDS read address per
-// slot.  Staging is pipelined: while a stage is summed, the raw windows of the
-// next stage and the record of the stage after it are loaded straight into LDS
-// (MUBUF loads with the LDS bit, no registers held, and no scalar loads, whose
-// out-of-order returns would force full LDS waits in the sums); at the stage
-// boundary the waves turn the raw bytes into the rows.
+// A planner kernel lays out each block as stages: slots (a slot is a channel
+// pair or a single channel) whose rows fit the LDS row area and whose raw
+// windows (one per channel) fit the raw area, taken from both ends of the band
+// in turn so that every stage mixes wide and narrow rows.  A stage is one
+// fixed-size record -- header, window channels and offsets, row records, every
+// trial's 16-bit LDS read address per slot and the row of every quad (four row
+// words).  Staging is pipelined: while a stage is summed, the raw windows of
+// the next stage and the record of the stage after it are loaded straight
+// into LDS (MUBUF loads with the LDS bit, no registers held, and no scalar
+// loads, whose out-of-order returns would force full LDS waits in the sums);
+// at the stage boundary all 512 lanes turn the raw bytes into the rows, one
+// quad per lane at a time.  Two workgroups share a CU, so one's row build
+// overlaps the other's sums.
 //
 // Workgroups are numbered XCD-aware: consecutive workgroup ids go to different
 // XCDs, so id is remapped so that each XCD walks a contiguous run of
@@ -187,32 +191,39 @@ constexpr int DD_TRIALS = DD_WAVES * DD_M;
 static_assert(DD_TRIALS == (int)DEDISP_BLOCK_TRIALS, "block size shared with the host");
 constexpr int DD_XCDS = 8;
 constexpr int DD_AHEAD = 2;  // trials whose LDS reads are issued ahead of their adds
-constexpr int DD_ROW_BYTES = 48 * 1024;  // rows of one stage (at most; two workgroups per CU up to 2048 channels)
+constexpr int DD_ROW_BYTES = 48 * 1024;  // rows of one stage (at most; two workgroups per CU)
 constexpr int DD_RAW_BYTES = 20 * 1024;  // raw byte windows of one stage (at most)
 constexpr int DD_RSRC_WORD3 = 0x00020000;  // buffer descriptor word 3: raw 32-bit data
-constexpr int DDP_MAX_WINDOWS = 20;  // wave w loads windows w, w + DD_WAVES, ...
-constexpr int DDP_MAX_ROWS = 3 * DD_WAVES;     // wave w builds rows w, w + 8, w + 16
-constexpr int DDP_MAX_SLOTS = 16;              // slots per stage
-constexpr int DDP_KMAX = 3;                    // pair rows per channel pair (at most)
-// stage record (u32 words): [0] nwin | nslot << 8 | nrow << 16, [4 ..] window
-// channels, [DDP_ROW_REC + 4 r ..] row r: rowoff | quads << 16, raw byte of
-// word L of channel c in its window, the same for channel c + 1, cwin | c1win
-// << 8 | paired << 16; [DDP_LAD_REC + 32 s ..] read addresses of slot s
-// (DD_TRIALS x u16)
-constexpr int DDP_ROW_REC = 4 + DDP_MAX_WINDOWS;
+constexpr int DD_RECS = 3;  // stage records held: summed, windows loading, record loading
+constexpr int DDP_MAX_WINDOWS = 20;
+constexpr int DDP_MAX_ROWS = 24;                  // rows per stage
+constexpr int DDP_MAX_SLOTS = 16;                 // slots per stage
+constexpr int DDP_KMAX = 3;                       // pair rows per channel pair (at most)
+// stage record (u32 words): [0] nwin | nslot << 8 | nrow << 16, [1] quads
+// (4-word units of the row buffer), [4 ..] window channels, [DDP_WIN_START ..]
+// window byte offsets from the tile start, [DDP_ROW_REC + 4 r ..] row r:
+// rowoff | quads << 16, raw byte of word L of channel c in its window, the same
+// for channel c + 1, cwin | c1win << 8 | paired << 16; [DDP_LAD_REC + 32 s ..]
+// read addresses of slot s (DD_TRIALS x u16); [DDP_QMAP_REC ..] the row of
+// every quad (u8).  Rows are packed in record order, so quad q of the stage is
+// quad q - rowoff / 64 of its row and lands at byte 64 q of the row buffer.
+constexpr int DDP_WIN_START = 4 + DDP_MAX_WINDOWS;
+constexpr int DDP_ROW_REC = DDP_WIN_START + DDP_MAX_WINDOWS;
 constexpr int DDP_LAD_REC = DDP_ROW_REC + 4 * DDP_MAX_ROWS;
-constexpr int DDP_STAGE_WORDS = DDP_LAD_REC + DDP_MAX_SLOTS * DD_TRIALS / 2;
-constexpr int DDP_STAGE_CHUNKS = (DDP_STAGE_WORDS + 63) / 64;  // 64-word loads
+constexpr int DDP_MAX_QUADS = DD_ROW_BYTES / 64;  // 4 row words of 16 bytes
+constexpr int DDP_QMAP_REC = DDP_LAD_REC + DDP_MAX_SLOTS * DD_TRIALS / 2;
+constexpr int DDP_STAGE_WORDS = DDP_QMAP_REC + DDP_MAX_QUADS / 4;
+constexpr int DDP_STAGE_CHUNKS = (DDP_STAGE_WORDS + 63) / 64;
 constexpr int DDP_STAGE_PITCH = 64 * DDP_STAGE_CHUNKS;
 constexpr int DDP_PLAN_HEAD = 64;  // word 0: stage count
 
-static size_t ddp_plan_words(uint32_t nchans) {  // per block; stages <= slots <= nchans
-    return DDP_PLAN_HEAD + (size_t)nchans * DDP_STAGE_PITCH;
+static size_t ddp_plan_shm(uint32_t nchans) {  // dynamic LDS of the planner
+    const size_t npair = (nchans + 1) / 2;
+    return 4 * (2 * (size_t)nchans + 2 * npair + 3 * DDP_KMAX * npair) + DDP_KMAX * npair;
 }
 
-// u16 pair of byte i (0..3) of dwords a (low lane) and b (high lane)
-__device__ __forceinline__ uint32_t pair_word(uint32_t b, uint32_t a, uint32_t i) {
-    return __builtin_amdgcn_perm(b, a, 0x0c040c00u + 0x00010001u * i);
+static size_t ddp_plan_words(uint32_t nchans) {  // per block; stages <= slots <= nchans
+    return DDP_PLAN_HEAD + (size_t)nchans * DDP_STAGE_PITCH;
 }
 
 // Raw window of channel ch in a tile starting at t0: bytes from
@@ -238,6 +249,7 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
     int32_t* udel = (int32_t*)(uslot + npair);
     uint32_t* uLW = (uint32_t*)(udel + (size_t)npair * DDP_KMAX);
     uint32_t* uRO = uLW + (size_t)npair * DDP_KMAX;
+    uint8_t* uRN = (uint8_t*)(uRO + (size_t)npair * DDP_KMAX);
     const uint32_t d0 = blk * DD_TRIALS;
     uint32_t* pb = plan + (size_t)blk * plan_words;
     auto delay = [&](uint32_t m, uint32_t c) {
@@ -306,7 +318,12 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
         }
     }
     __syncthreads();
-    // stages, in channel order (one thread; each pair's data is read up front)
+    // stages (one thread; each pair's data is read up front).  Pairs are taken
+    // from both ends of the band in turn (0, npair - 1, 1, npair - 2, ...):
+    // the wide rows of the long-delay channels and the many narrow ones of the
+    // short-delay channels then share every stage, which keeps the row build
+    // of each stage and the sums of the one before it of similar length (the
+    // integer sums are exact in any channel order).
     if (threadIdx.x == 0) {
         uint32_t nst = 0, nslot_all = 0;
         uint32_t nwin = 0, nslot = 0, nrow = 0, bytes = 0;
@@ -314,6 +331,7 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
         auto close = [&]() {
             if (!nslot) return;
             st[0] = nwin | nslot << 8 | nrow << 16;
+            st[1] = bytes / 64;
             nst++;
             st = pb + DDP_PLAN_HEAD + (size_t)nst * DDP_STAGE_PITCH;
             nwin = nslot = nrow = bytes = 0;
@@ -321,6 +339,7 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
         auto put_row = [&](uint32_t ui, uint32_t W, uint32_t offc, uint32_t offc1, uint32_t cw, uint32_t c1w,
                            uint32_t paired) {
             uRO[ui] = bytes;
+            uRN[ui] = (uint8_t)nrow;
             uint32_t* r = st + DDP_ROW_REC + 4 * nrow;
             r[0] = bytes | (W / 4) << 16;
             r[1] = offc;
@@ -338,7 +357,8 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
             return nwin + w <= max_windows && nrow + r <= (uint32_t)DDP_MAX_ROWS && bytes + b <= row_budget &&
                    nslot < (uint32_t)DDP_MAX_SLOTS;
         };
-        for (uint32_t u = 0; u < npair; u++) {
+        for (uint32_t i = 0; i < npair; i++) {
+            const uint32_t u = i & 1 ? npair - 1 - i / 2 : i / 2;
             const uint32_t c = 2 * u, K = ukind[u];
             uint32_t LW[DDP_KMAX];
             int32_t dl[DDP_KMAX];
@@ -356,6 +376,8 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
                 uslot[u] = nslot_all;
                 st[4 + nwin] = c;
                 st[5 + nwin] = c + 1;
+                st[DDP_WIN_START + nwin] = (uint32_t)((sb0 & ~15) - 16);
+                st[DDP_WIN_START + 1 + nwin] = (uint32_t)((sb1 & ~15) - 16);
 #pragma unroll
                 for (int k = 0; k < DDP_KMAX; k++) {
                     if ((uint32_t)k >= K) break;
@@ -371,6 +393,7 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
                     if (!fits(1, 1, 16 * W)) close();
                     if (j == 0) uslot[u] = nslot_all;
                     st[4 + nwin] = c + j;
+                    st[DDP_WIN_START + nwin] = (uint32_t)(((j ? sb1 : sb0) & ~15) - 16);
                     put_row(u * DDP_KMAX + j, W, ddp_mis(j ? sb1 : sb0) + L, 0, nwin, 0, 0);
                     nwin++;
                     put_slot();
@@ -381,6 +404,18 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
         pb[0] = nst;
     }
     __syncthreads();
+    // the row of every quad of every stage: one pair per thread
+    for (uint32_t u = threadIdx.x; u < npair; u += blockDim.x) {
+        const uint32_t c = 2 * u, K = ukind[u];
+        const uint32_t nr = K ? K : (c + 1 < nchans ? 2u : 1u);
+        for (uint32_t k = 0; k < nr; k++) {
+            const uint32_t ui = u * DDP_KMAX + k;
+            const uint32_t stg = sloc[uslot[u] + (K ? 0u : k)] & 0xffffu;
+            uint8_t* map = (uint8_t*)(pb + DDP_PLAN_HEAD + (size_t)stg * DDP_STAGE_PITCH + DDP_QMAP_REC);
+            const uint32_t q0 = uRO[ui] / 64, nq = (uLW[ui] >> 16) / 4;
+            for (uint32_t q = 0; q < nq; q++) map[q0 + q] = uRN[ui];
+        }
+    }
     // every trial's LDS read address per slot: one pair per thread, 16 trials
     // at a time
     auto lad = [&](uint32_t slot) {
@@ -431,25 +466,24 @@ __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ del
 }
 
 // One work item of the u8 dedispersion: DM block `blk` (trials blk * 64 ..) over
-// time tile `tile` (outputs tile * 512 ..).  bases/plan are the whole launch's
-// tables (indexed by blk), partial has ntiles words per trial.
+// time tile `tile` (outputs tile * 512 ..).  plan is the whole launch's table
+// (indexed by blk), partial has ntiles words per trial.
 __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64_t pitch,
-                                        uint32_t nchans, uint32_t phase_slots,
-                                        const int32_t* __restrict__ bases,
-                                        const uint32_t* __restrict__ plan, uint64_t plan_words,
-                                        uint32_t ntrials, uint32_t ntiles, uint32_t n,
+                                        uint32_t phase_slots, const uint32_t* __restrict__ plan,
+                                        uint64_t plan_words, uint32_t ntrials, uint32_t ntiles, uint32_t n,
                                         void* __restrict__ out, uint32_t out_pitch, int fp16, float m0,
                                         float inv_s0, double* __restrict__ partial, uint32_t RB,
                                         uint32_t tile, uint32_t blk) {
-    // Static LDS: the stage's rows, three stage records (the current one, the
-    // next, whose raw windows are loaded during this stage, and the one after,
-    // loaded now) and the raw windows [window][RB]; records and windows arrive
-    // by direct global->LDS loads into separate LDS objects, so the sums'
-    // reads of the rows need not wait for them.  Dynamic LDS: channel bases.
-    __shared__ __align__(16) uint32_t lds[DD_ROW_BYTES / 4];
-    __shared__ __align__(16) uint32_t srec[3][DDP_STAGE_PITCH];
+    // LDS: the raw windows [window][RB], the stage's rows and three stage
+    // records (the current one, the next, whose raw windows are loaded during
+    // this stage, and the one after).  Windows arrive by direct global->LDS
+    // loads into their own LDS object, which the sums never read, so no read
+    // of the sums waits for them; records go through registers (loaded during
+    // one stage's sums, stored at the next stage boundary) for the same
+    // reason: the sums read their LDS addresses from the records.
     __shared__ __align__(16) uint8_t raw[DD_RAW_BYTES];
-    extern __shared__ __align__(16) int32_t lbase[];
+    __shared__ __align__(16) uint32_t lds[DD_ROW_BYTES / 4];
+    __shared__ __align__(16) uint32_t srec[DD_RECS][DDP_STAGE_PITCH];
 
     const uint32_t t0 = tile * DD_TILE;
     const uint32_t d0 = blk * DD_TRIALS;
@@ -483,31 +517,31 @@ __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64
     };
     uint32_t in_phase = 0;
     const char* lds_lane = (const char*)lds + 16 * (threadIdx.x & 63);
-    for (uint32_t i = threadIdx.x; i < nchans; i += DD_THREADS) lbase[i] = bases[(size_t)blk * nchans + i];
-    // record of stage st into srec[st % 3], 64 words per wave and load
-    auto load_record = [&](uint32_t st) {
-        uint32_t ln = threadIdx.x;  // recomputed per call: hoisted per-lane values would spill
-        asm volatile("" : "+v"(ln));
-        ln &= 63;
-        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(pb + DDP_PLAN_HEAD + (size_t)st * DDP_STAGE_PITCH), 0, 4 * DDP_STAGE_PITCH, DD_RSRC_WORD3);
-        for (int ck = wave; ck < DDP_STAGE_CHUNKS; ck += DD_WAVES)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, srec[st % 3] + 64 * ck, 4, 4 * (64 * ck + ln), 0, 0, 0);
+    // record of stage st: thread t holds dwords 2 t and 2 t + 1 (indices
+    // clamped into range; the surplus is never stored)
+    auto rec_load = [&](uint32_t st) {
+        const uint32_t* src = pb + DDP_PLAN_HEAD + (size_t)st * DDP_STAGE_PITCH;
+        const uint32_t i = min(2 * (uint32_t)threadIdx.x, (uint32_t)DDP_STAGE_PITCH - 2);
+        return make_uint2(src[i], src[i + 1]);
     };
-    // raw windows of stage st: wave w loads windows w and w + DD_WAVES (lanes
+    auto rec_store = [&](uint32_t st, uint2 v) {
+        if (2 * threadIdx.x < (uint32_t)DDP_STAGE_PITCH) *(uint2*)(srec[st % DD_RECS] + 2 * threadIdx.x) = v;
+    };
+    static_assert(2 * DD_THREADS >= DDP_STAGE_PITCH, "a record is two dwords per thread");
+    // raw windows of stage st: wave w loads windows w, w + 8 and w + 16 (lanes
     // whose 16 bytes fall outside the row read its nearest 16 in-row bytes
     // instead: those samples only reach outputs at or past n, or words no
     // trial reads)
     auto load_windows = [&](uint32_t st) {
-        const uint32_t* sr = srec[st % 3];
+        const uint32_t* sr = srec[st % DD_RECS];
         const uint32_t nwin = __builtin_amdgcn_readfirstlane(sr[0]) & 0xffu;
         uint32_t ln = threadIdx.x;
         asm volatile("" : "+v"(ln));
         ln &= 63;
         for (uint32_t wi = (uint32_t)wave; wi < nwin; wi += DD_WAVES) {
             const uint32_t c = __builtin_amdgcn_readfirstlane(sr[4 + wi]);
-            // 32-bit offsets: t0 + base < pitch < 2^31
-            const int32_t start = (int32_t)t0 + (lbase[c] & ~15) - 16;
+            // 32-bit offsets: t0 + start < pitch < 2^31
+            const int32_t start = (int32_t)t0 + (int32_t)__builtin_amdgcn_readfirstlane(sr[DDP_WIN_START + wi]);
             const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
                 (void*)(data + (size_t)c * pitch), 0, (int)pitch, DD_RSRC_WORD3);
             for (uint32_t k = 0; k < RB; k += 1024) {
@@ -517,75 +551,67 @@ __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64
             }
         }
     };
-    load_record(0);
-    if (nstages > 1) load_record(1);
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    __syncthreads();
-    load_windows(0);
-    for (uint32_t st = 0; st < nstages; st++) {
-        const uint32_t* sr = srec[st % 3];
-        const uint32_t h = __builtin_amdgcn_readfirstlane(sr[0]);
-        const uint32_t nslot = (h >> 8) & 0xffu, nrow = h >> 16;
-        // this wave's loads (stage st's windows, stage st + 1's record) have
-        // landed; after the barrier every wave's have, and the previous stage's
-        // sums are done with the rows
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-        __syncthreads();
-        // rows: wave w builds rows w, w + 8, w + 16.  Lane g builds words
-        // L + 4g .. L + 4g + 3 from dword g + 16 k (k < 8) of the windows; its
-        // four 16-byte stores go in a lane-rotated order, which spreads every
-        // 16 lanes' stores over all LDS banks.
-        for (uint32_t rr = (uint32_t)wave; rr < nrow; rr += DD_WAVES) {
-            uint32_t ln = threadIdx.x;  // as in load_record
-            asm volatile("" : "+v"(ln));
-            ln &= 63;
-            const uint32_t* rec = sr + DDP_ROW_REC + 4 * rr;
-            const uint32_t r0 = __builtin_amdgcn_readfirstlane(rec[0]);
-            const uint32_t offc = __builtin_amdgcn_readfirstlane(rec[1]);
-            const uint32_t offc1 = __builtin_amdgcn_readfirstlane(rec[2]);
-            const uint32_t r3 = __builtin_amdgcn_readfirstlane(rec[3]);
-            const uint32_t quads = r0 >> 16;
-            uint32_t* row = lds + (r0 & 0xffffu) / 4;
-            const uint32_t* rc = (const uint32_t*)(raw + (size_t)(r3 & 0xffu) * RB + offc);
-            const uint32_t rot = (ln >> 2) & 3u;
-            if (!(r3 >> 16)) {
-                for (uint32_t g = ln; g < quads; g += 64) {
-                    uint32_t d[8];
+    // rows of stage st: the 512 lanes take the stage's quads (4 row words) in
+    // turn; quad q belongs to row qmap[q] and lands at row-buffer word 16 q.  A
+    // lane builds its quad's words from dword g + 16 k (k < 8) of the windows
+    // (g = the quad's index in its row) and stores them in a lane-rotated
+    // order, which spreads every 16 lanes' stores over all LDS banks.
+    // Single-channel rows read their own window twice and zero the second
+    // channel's bytes with an all-zero byte select.
+    auto build = [&](uint32_t st) {
+        const uint32_t* sr = srec[st % DD_RECS];
+        const uint32_t nq = __builtin_amdgcn_readfirstlane(sr[1]);
+        const uint8_t* qmap = (const uint8_t*)(sr + DDP_QMAP_REC);
+        for (uint32_t q = threadIdx.x; q < nq; q += DD_THREADS) {
+            const uint32_t r = qmap[q];
+            const uint4 rec = *(const uint4*)(sr + DDP_ROW_REC + 4 * r);
+            const uint32_t g = q - (rec.x & 0xffffu) / 64;
+            const bool paired = (rec.w >> 16) & 1u;
+            const uint32_t* rc = (const uint32_t*)(raw + (rec.w & 0xffu) * RB + rec.y) + g;
+            const uint32_t* r1 =
+                paired ? (const uint32_t*)(raw + ((rec.w >> 8) & 0xffu) * RB + (rec.z & ~3u)) + g : rc;
+            const uint32_t sh = paired ? rec.z & 3u : 0u;
+            uint32_t d[8], e[8];
 #pragma unroll
-                    for (int k = 0; k < 8; k++) d[k] = rc[g + 16 * k];
+            for (int k = 0; k < 8; k++) {
+                d[k] = rc[16 * k];
+                e[k] = __builtin_amdgcn_alignbyte(r1[16 * k + 1], r1[16 * k], sh);
+            }
+            const uint32_t rot = (q >> 2) & 3u;
+            uint32_t* row = lds + 16 * q;
 #pragma unroll
-                    for (uint32_t bb = 0; bb < 4; bb++) {
-                        const uint32_t b = (bb + rot) & 3u;
-                        *(uint4*)(row + 4 * (4 * g + b)) = make_uint4(
-                            pair_word(d[1], d[0], b), pair_word(d[3], d[2], b), pair_word(d[5], d[4], b),
-                            pair_word(d[7], d[6], b));
-                    }
-                }
-            } else {
-                const uint32_t* r1 = (const uint32_t*)(raw + (size_t)((r3 >> 8) & 0xffu) * RB + (offc1 & ~3u));
-                const uint32_t sh = offc1 & 3u;
-                for (uint32_t g = ln; g < quads; g += 64) {
-                    uint32_t d[8], e[8];
-#pragma unroll
-                    for (int k = 0; k < 8; k++) {
-                        d[k] = rc[g + 16 * k];
-                        e[k] = __builtin_amdgcn_alignbyte(r1[g + 16 * k + 1], r1[g + 16 * k], sh);
-                    }
-#pragma unroll
-                    for (uint32_t bb = 0; bb < 4; bb++) {
-                        const uint32_t b = (bb + rot) & 3u;
-                        *(uint4*)(row + 4 * (4 * g + b)) = make_uint4(
-                            pair_word(d[1], d[0], b) + pair_word(e[1], e[0], b),
-                            pair_word(d[3], d[2], b) + pair_word(e[3], e[2], b),
-                            pair_word(d[5], d[4], b) + pair_word(e[5], e[4], b),
-                            pair_word(d[7], d[6], b) + pair_word(e[7], e[6], b));
-                    }
-                }
+            for (uint32_t bb = 0; bb < 4; bb++) {
+                const uint32_t b = (bb + rot) & 3u;
+                const uint32_t sd = 0x0c040c00u + 0x00010001u * b, se = paired ? sd : 0x0c0c0c0cu;
+                *(uint4*)(row + 4 * b) = make_uint4(
+                    __builtin_amdgcn_perm(d[1], d[0], sd) + __builtin_amdgcn_perm(e[1], e[0], se),
+                    __builtin_amdgcn_perm(d[3], d[2], sd) + __builtin_amdgcn_perm(e[3], e[2], se),
+                    __builtin_amdgcn_perm(d[5], d[4], sd) + __builtin_amdgcn_perm(e[5], e[4], se),
+                    __builtin_amdgcn_perm(d[7], d[6], sd) + __builtin_amdgcn_perm(e[7], e[6], se));
             }
         }
+    };
+    {
+        const uint2 r0 = rec_load(0), r1 = rec_load(min(1u, nstages - 1));  // nstages >= 1: nchans >= 1
+        rec_store(0, r0);
+        if (nstages > 1) rec_store(1, r1);
+    }
+    __syncthreads();
+    load_windows(0);
+    uint2 rnext = make_uint2(0u, 0u);  // record of stage st + 1, loaded during stage st - 1
+    for (uint32_t st = 0; st < nstages; st++) {
+        const uint32_t* sr = srec[st % DD_RECS];
+        const uint32_t nslot = (__builtin_amdgcn_readfirstlane(sr[0]) >> 8) & 0xffu;
+        // this wave's loads (stage st's windows, stage st + 1's record) have
+        // landed; after the barrier every wave's have, and the previous stage's
+        // sums are done with the rows and with stage st - 1's record slot
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __syncthreads();
+        if (st >= 1 && st + 1 < nstages) rec_store(st + 1, rnext);
+        build(st);
         __syncthreads();
         if (st + 1 < nstages) load_windows(st + 1);
-        if (st + 2 < nstages) load_record(st + 2);
+        rnext = rec_load(min(st + 2, nstages - 1));
         // slots in pairs: one v_add3_u32 adds both slots' u16 lanes (no lane can
         // carry within a phase: every slot adds at most 2 * maxval per lane).
         // The wave's 8 read addresses per slot come from 16-byte broadcast reads
@@ -666,7 +692,7 @@ __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64
             acc32[m][2 * j + 1] = (lo[m][j] >> 16) + (((cy[m] >> (4 * j + 16)) & 15u) << 16);
         }
     const int m0i = (int)m0;
-    uint32_t ln = threadIdx.x;  // as in load_record
+    uint32_t ln = threadIdx.x;
     asm volatile("" : "+v"(ln));
     ln &= 63;
 #pragma unroll
@@ -703,13 +729,15 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
     const int32_t* __restrict__ bases, const uint32_t* __restrict__ plan, uint64_t plan_words,
     uint32_t ntrials, uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out,
     uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t RB) {
+    (void)nchans;
+    (void)bases;
     const uint32_t total = ntiles * nblk;
     const uint32_t id = blockIdx.x;
     const uint32_t xcd = id % DD_XCDS, local = id / DD_XCDS;
     const uint32_t q = total / DD_XCDS, r = total % DD_XCDS;
     const uint32_t work = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
-    dd_item(data, pitch, nchans, phase_slots, bases, plan, plan_words, ntrials, ntiles, n, out, out_pitch,
-            fp16, m0, inv_s0, partial, RB, work / nblk, work % nblk);
+    dd_item(data, pitch, phase_slots, plan, plan_words, ntrials, ntiles, n, out, out_pitch, fp16, m0, inv_s0,
+            partial, RB, work / nblk, work % nblk);
 }
 
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
@@ -739,11 +767,8 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
     while (kmax > 1 && (uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax--;
     if ((uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax = 0;
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
-    const size_t npair = (nchans + 1) / 2;
-    const size_t plan_shm = 4 * (2 * (size_t)nchans + 2 * npair + 3 * DDP_KMAX * npair);
+    const size_t plan_shm = ddp_plan_shm(nchans);
     if (plan_shm > 160 * 1024) return false;
-    const size_t shm = 4 * (size_t)nchans;
-    if (shm > 64 * 1024) return false;
     int32_t* bases = table;
     uint32_t* plan = (uint32_t*)(table + nblk * nchans);
     const uint64_t pw = ddp_plan_words(nchans);
@@ -756,11 +781,9 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
     uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
     phase = phase > 4096 ? 4096 : phase;
     if (phase < 2) return false;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dedisp_u8),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     const uint32_t ntiles = (n + DD_TILE - 1) / DD_TILE;
     const uint64_t total = (uint64_t)ntiles * nblk;
-    hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(DD_THREADS), shm, s, data, pitch, nchans,
+    hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(DD_THREADS), 0, s, data, pitch, nchans,
                        phase, bases, plan, pw, ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16,
                        m0, inv_s0, partial, RB);
     return true;
@@ -845,7 +868,7 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_queue(const DedispJob*
             const uint32_t tile = tg * span + k;
             if (tile >= j.ntiles) break;
             if (k) __syncthreads();
-            dd_item(j.data, j.pitch, j.nchans, j.phase_slots, j.bases, j.plan, j.plan_words, j.ntrials,
+            dd_item(j.data, j.pitch, j.phase_slots, j.plan, j.plan_words, j.ntrials,
                     j.ntiles, j.n, j.out, j.out_pitch, fp16, j.m0, j.inv_s0, j.partial, j.RB, tile, blk);
         }
     }
@@ -860,17 +883,13 @@ void launch_dedisp_batch(hipStream_t s, const DedispJob* jobs, const uint32_t* e
         hipLaunchKernelGGL(k_delays_multi, dim3((total_entries + 255) / 256), dim3(256), 0, s, jobs, entry0,
                            njobs, total_entries);
     if (!total_blocks || !total_items || !grid) return;
-    const size_t npair = (max_nchans + 1) / 2;
-    const size_t plan_shm = 4 * (2 * (size_t)max_nchans + 2 * npair + 3 * DDP_KMAX * npair);
+    const size_t plan_shm = ddp_plan_shm(max_nchans);
     if (plan_shm > 64 * 1024)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dd_plan_multi),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan_shm);
     hipLaunchKernelGGL(k_dd_plan_multi, dim3(total_blocks), dim3(256), plan_shm, s, jobs, blk0, njobs);
     launch_fill_u32(s, counters, DD_XCDS, 0u);
-    const size_t shm = 4 * (size_t)max_nchans;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dedisp_queue),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL(k_dedisp_queue, dim3(grid), dim3(DD_THREADS), shm, s, jobs, item0, njobs,
+    hipLaunchKernelGGL(k_dedisp_queue, dim3(grid), dim3(DD_THREADS), 0, s, jobs, item0, njobs,
                        total_items, span, fp16, counters);
 }
 
@@ -888,9 +907,7 @@ bool dedisp_job_setup(uint32_t nchans, uint32_t maxval, uint64_t pitch, const ui
     uint32_t kmax = DDP_KMAX;
     while (kmax > 1 && (uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax--;
     if ((uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax = 0;
-    const size_t npair = (nchans + 1) / 2;
-    if (4 * (2 * (size_t)nchans + 2 * npair + 3 * DDP_KMAX * npair) > 160 * 1024) return false;
-    if (4 * (size_t)nchans > 64 * 1024) return false;
+    if (ddp_plan_shm(nchans) > 160 * 1024) return false;
     uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
     phase = phase > 4096 ? 4096 : phase;
     if (phase < 2) return false;

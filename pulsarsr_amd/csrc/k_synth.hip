@@ -5,7 +5,8 @@
 // broadband bursts and Gaussian pulses delayed by the fractional physical
 // dispersion delay -- quantised to round(mean + sigma * value) clipped to the
 // bit depth and packed LSB-first in time-major frames.  One thread per
-// payload byte (it computes the byte's 8 / nbits samples), so sub-byte frames
+// payload byte at a time (it computes the byte's 8 / nbits samples; a
+// grid-stride loop covers payloads of 2^32 bytes and more), so sub-byte frames
 // pack without races.  A sample's pulse term counts every pulse whose centre
 // is within 5 sigma of phase, which the host generator also uses as its
 // support; rounding near a quantisation edge may still differ from the host
@@ -13,6 +14,7 @@
 // model, each deterministic in its seed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -68,13 +70,9 @@ __device__ __forceinline__ float norm_sample(const double* __restrict__ q, uint6
     return (float)(q[i] + f * (q[i + 1] - q[i]));
 }
 
-__global__ __launch_bounds__(256) void k_synth(DevCfg c, const double* __restrict__ q,
-                                               const double* __restrict__ delay,  // [nchans] per pulsar
-                                               const uint8_t* __restrict__ is_rfi,
-                                               const float* __restrict__ common, uint8_t* __restrict__ out,
-                                               uint64_t nbytes) {
-    const uint64_t byte = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (byte >= nbytes) return;
+__device__ void synth_byte(const DevCfg& c, const double* __restrict__ q, const double* __restrict__ delay,
+                           const uint8_t* __restrict__ is_rfi, const float* __restrict__ common,
+                           uint8_t* __restrict__ out, uint64_t byte) {
     const uint32_t per = c.nbits >= 8 ? 1u : 8u / c.nbits;
     uint32_t packed = 0;
     for (uint32_t s = 0; s < per; s++) {
@@ -104,6 +102,17 @@ __global__ __launch_bounds__(256) void k_synth(DevCfg c, const double* __restric
         }
     }
     out[byte] = (uint8_t)packed;
+}
+
+__global__ __launch_bounds__(256) void k_synth(DevCfg c, const double* __restrict__ q,
+                                               const double* __restrict__ delay,  // [nchans] per pulsar
+                                               const uint8_t* __restrict__ is_rfi,
+                                               const float* __restrict__ common, uint8_t* __restrict__ out,
+                                               uint64_t nbytes) {
+    // grid-stride: payloads of 4 GiB and more exceed one launch's 2^32 threads
+    for (uint64_t byte = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; byte < nbytes;
+         byte += (uint64_t)gridDim.x * blockDim.x)
+        synth_byte(c, q, delay, is_rfi, common, out, byte);
 }
 
 std::vector<double> make_table() {
@@ -190,8 +199,8 @@ extern "C" int psr_synth_generate_gpu(const psr_synth_config* c, uint8_t* host_o
     if (e == hipSuccess) e = hipMemcpy(dr, rfi.data(), rfi.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && dc) e = hipMemcpy(dc, c->common, c->nsamples * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_synth, dim3((unsigned)((need + 255) / 256)), dim3(256), 0, 0, d, dq, dd, dr, dc, dout,
-                           need);
+        const uint64_t blocks = std::min<uint64_t>((need + 255) / 256, 1u << 20);
+        hipLaunchKernelGGL(k_synth, dim3((unsigned)blocks), dim3(256), 0, 0, d, dq, dd, dr, dc, dout, need);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(host_out, dout, need, hipMemcpyDeviceToHost);
