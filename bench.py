@@ -79,10 +79,10 @@ def top_families(cands, n, tsamp, k=20):
 # that launch shape only: other workloads or trial counts report traffic null.
 MEASURED_LAUNCH = {"workload": "headline-1024ch-8bit", "trials": 4000, "launches": 1}
 MEASURED_TRAFFIC_GB_PER_LAUNCH = {
-    "k_dedisp_u8": 91.40 + 17.14,
+    "k_dedisp_u8": 105.03 + 17.65,
     "k_fft_pass1r": 16.78 + 67.11,
     "k_fft_pass2r": 67.25 + 16.81,
-    "k_whiten_fused16": 31.77 + 17.35,
+    "k_whiten_fused16": 31.93 + 17.36,
     "k_harmsum": 15.22 + 0.02,
 }
 
