@@ -371,8 +371,9 @@ def main():
                               "frac": ach_tadds / VALU_PEAK_TADDS},
                      "note": "1 LDS byte per add (channel-pair rows), adds = trials x series_len x nchans; "
                              "HIP-event time of the dedispersion stage; traffic = HBM bytes of this launch "
-                             "(FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_headline-1024ch-8bit.md): the "
-                             "filterbank is re-read from the fabric about 21 times per launch, 11.0 of 13.1 B/unit",
+                             "(FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_headline-1024ch-8bit.md): 12.6 of "
+                             "14.8 B/unit are reads, the stage records (0.24 MB per work item from the 0.23 GB "
+                             "plan table) and filterbank windows fetched again after L2 eviction",
                      "hbm": {"achieved": (units * 2.0 + w.nchans * w.nsamples) / (dd_ms * 1e-3) / 1e9,
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": (units * 2.0 + w.nchans * w.nsamples) / (dd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
