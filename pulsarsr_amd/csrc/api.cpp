@@ -104,7 +104,7 @@ namespace {
 
 int upload_twiddles(psr_ctx* c, const SpecGeom& g) {
     if (!c->costabs.p) {
-        std::vector<float> ct(COSTAB_SIZE);
+        std::vector<float> ct(COSTAB_ALL);
         fill_costabs(ct.data());
         int rc = ensure(c, c->costabs, ct.size() * sizeof(float));
         if (rc) return rc;

@@ -661,7 +661,7 @@ int psr_batch_create(psr_ctx* c, psr_obs* const* obs, size_t nobs, const double*
     }
     if ((rc = alloc_buffers(b))) return batch_fail_cleanup(b, rc);
     {
-        std::vector<float> ct(COSTAB_SIZE);
+        std::vector<float> ct(COSTAB_ALL);
         fill_costabs(ct.data());
         if ((rc = ensure(c, b->costabs, ct.size() * sizeof(float)))) return batch_fail_cleanup(b, rc);
         if (hipMemcpy(b->costabs.p, ct.data(), ct.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)

@@ -155,7 +155,12 @@ __host__ __device__ constexpr int costab_offset(int L) {
     return L <= 2 ? 0 : costab_offset(L / 2) + (L / 2) / 4 + 1;
 }
 constexpr int COSTAB_SIZE = costab_offset(2 * COSTAB_MAX);
-void fill_costabs(float* host_table);
+// After the cosine tables (8-byte aligned): W_4096^(k t) as (re, im) float
+// pairs at float index TW4096_OFFSET + 2 (256 (k - 1) + t), k = 1..15, t < 256
+// (the twiddles of the first stage of the length-4096 row FFTs).
+constexpr int TW4096_OFFSET = (COSTAB_SIZE + 1) & ~1;
+constexpr int COSTAB_ALL = TW4096_OFFSET + 2 * 15 * 256;
+void fill_costabs(float* host_table);  // COSTAB_ALL floats
 // Intermediate bytes per four-step FFT batch (the work buffer holds this many
 // trials' intermediates): 192 MiB (11 trials at 2^22) stays within the 256 MiB
 // Infinity Cache between the two passes; measured against 128 MiB to 1 GiB

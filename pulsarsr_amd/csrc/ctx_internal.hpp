@@ -101,7 +101,7 @@ struct SpecBufs {
     CandidateOut* cands = nullptr;    // [nt][cap_per_trial]
     uint32_t* counts = nullptr;       // [nt + 1]
     const float2* twiddles = nullptr;  // twiddle_table_size(g)
-    const float* costabs = nullptr;    // COSTAB_SIZE
+    const float* costabs = nullptr;    // COSTAB_ALL
 };
 size_t spectral_work_bytes(const SpecGeom& g, uint32_t nt);
 // FFT, whitening, harmonic sums and ranking on stream s; allocates nothing

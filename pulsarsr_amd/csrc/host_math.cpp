@@ -289,6 +289,13 @@ void fill_costabs(float* t) {
         for (int j = 0; j <= L / 4; j++) t[off + j] = (float)std::cos(2.0 * M_PI * (double)j / (double)L);
         t[off + L / 4] = 0.0f;  // cos(pi/2)
     }
+    for (int i = COSTAB_SIZE; i < TW4096_OFFSET; i++) t[i] = 0.0f;
+    for (int k = 1; k < 16; k++)
+        for (int j = 0; j < 256; j++) {
+            const double a = -2.0 * M_PI * (double)((k * j) % 4096) / 4096.0;
+            t[TW4096_OFFSET + 2 * (256 * (k - 1) + j)] = (float)std::cos(a);
+            t[TW4096_OFFSET + 2 * (256 * (k - 1) + j) + 1] = (float)std::sin(a);
+        }
 }
 
 }  // namespace psr

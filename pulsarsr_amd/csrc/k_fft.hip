@@ -692,16 +692,19 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
     const float2* w = work + (size_t)trial * M;
     const int t = threadIdx.x;
     float2 v[16];
-    {  // stage A: radix 16 over m, twiddle W_4096^(tt k1)
+    {  // stage A: radix 16 over m, twiddle W_4096^(tt k1) from the FP64-made table
         const int row = t >> 8, tt = t & 255;
         const float2* src = w + (size_t)(row ? r2 : r1) * M1 + tt;
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = src[256 * m];
+        const float2* T = (const float2*)(cos_tables + TW4096_OFFSET) + tt;
+        float2 tw[15];
+#pragma unroll
+        for (int k = 0; k < 15; k++) tw[k] = T[256 * k];
         if (t < 256) TA[t] = twiddle<256>(cos_tables + costab_offset(256), (t >> 4) * (t & 15));
-        const float* C4096 = cos_tables + costab_offset(4096);
-        const float2 w1 = twiddle<4096>(C4096, tt), w4 = twiddle<4096>(C4096, 4 * tt);
         dft<16>(v);
-        apply_tw<16>(v, w1, w4);
+#pragma unroll
+        for (int k = 1; k < 16; k++) v[k] = cmul(v[k], tw[k - 1]);
 #pragma unroll
         for (int k = 0; k < 16; k++) buf[row * RS + k * JP + (tt >> 4) * 16 + (tt & 15)] = v[k];
     }

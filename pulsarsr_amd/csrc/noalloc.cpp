@@ -79,7 +79,7 @@ Layout layout_of(const SpecGeom& g, size_t cap) {
     L.cands = take(cap * sizeof(CandidateOut));
     L.counts = take(2 * sizeof(uint32_t));
     L.twiddles = take(twiddle_table_size(g) * sizeof(float2));
-    L.costabs = take((size_t)COSTAB_SIZE * sizeof(float));
+    L.costabs = take((size_t)COSTAB_ALL * sizeof(float));
     L.total = off;
     return L;
 }
@@ -122,7 +122,7 @@ extern "C" int psr_periodicity_noalloc(psr_ctx* c, const float* series, size_t n
     const float head[2] = {(float)mean, 0.0f};  // mean32, corr16
     std::vector<float2> tw(twiddle_table_size(g));
     fill_twiddles(g, tw.data());
-    std::vector<float> ct(COSTAB_SIZE);
+    std::vector<float> ct(COSTAB_ALL);
     fill_costabs(ct.data());
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(base + L.series, series, n * sizeof(float), hipMemcpyHostToDevice, s));
