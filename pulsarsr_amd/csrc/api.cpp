@@ -100,6 +100,18 @@ std::vector<uint32_t> psr::spread_bounds(const psr_obs* o, const double* dms_sor
     return out;
 }
 
+uint64_t psr::max_delay_at(const psr_obs* o, double dm) {
+    // the device table's expression (k_delays: relative to the observation's
+    // reference frequency, also for channel-range views), plus one
+    double worst = 0.0;
+    for (const double f : o->freqs) {
+        const double samples = std::round(DISPERSION_SECONDS_MHZ * dm * (1.0 / (f * f) - o->ref_inv) / o->tsamp);
+        if (!(samples <= 4e18)) return UINT64_MAX;
+        worst = std::max(worst, samples);
+    }
+    return (uint64_t)worst + 1u;
+}
+
 namespace {
 
 int upload_twiddles(psr_ctx* c, const SpecGeom& g) {
@@ -509,7 +521,7 @@ int psr_obs_create(psr_ctx* c, const psr_filterbank* fb, psr_obs** out) {
     o->maxval = nb <= 8 ? (1u << nb) - 1u : 0u;
     const size_t es = o->kind == KIND_U8 ? 1 : (o->kind == KIND_U16 ? 2 : 4);
     o->pitch = ((ns + 255) & ~(uint64_t)255) + 256;
-    const size_t bytes = (size_t)o->pitch * o->nchans * es;
+    const size_t bytes = (size_t)o->pitch * o->nchans * es + (es == 1 ? DD_ALLOC_SLACK : 0);
     const size_t pbytes = (size_t)((ns * frame_bits + 7) / 8);
     void* dpay = nullptr;
     hipError_t e = hipMalloc(&o->data, bytes);
@@ -564,7 +576,7 @@ int psr_obs_create_f32(psr_ctx* c, const float* data, uint64_t nsamples, uint32_
     o->maxval = small_int ? std::max<uint32_t>(1u, (uint32_t)vmax) : 0u;
     const size_t es = small_int ? 1 : 4;
     o->pitch = ((nsamples + 255) & ~(uint64_t)255) + 256;
-    const size_t bytes = (size_t)o->pitch * nchans * es;
+    const size_t bytes = (size_t)o->pitch * nchans * es + (small_int ? DD_ALLOC_SLACK : 0);
     hipError_t e = hipMalloc(&o->data, bytes);
     if (e == hipSuccess) e = hipMemset(o->data, 0, bytes);
     if (e == hipSuccess) {
@@ -660,7 +672,7 @@ static int dedisperse_into(psr_ctx* c, psr_obs* o, const SpecGeom& g, const Slot
         int_done = launch_dedisp_u8(s, (const uint8_t*)o->data, o->pitch, o->nchans, o->maxval,
                                     (const int32_t*)c->delays.p, (int32_t*)c->dd_table.p, nt, n,
                                     sl.series->p, g.series_pitch, fp16, (float)o->m0, (float)inv_s0,
-                                    (double*)c->dd_partial.p, spread.data());
+                                    (double*)c->dd_partial.p, spread.data(), max_delay_at(o, dms_host[nt - 1]));
     }
     if (!int_done) {
         launch_dedisp_f32(s, o->data, o->kind, o->pitch, o->nchans, (const int32_t*)c->delays.p, nt,

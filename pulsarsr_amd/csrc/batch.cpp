@@ -47,7 +47,7 @@ struct ObsPlan {
     std::vector<uint32_t> birdies;
     int cls = -1;
     bool u8 = false;                  // LDS-staged kernel (else channel-ordered f32 kernel)
-    DedispJob proto{};                // RB, windows, kmax, phase_slots, plan_words when u8
+    DedispJob proto{};                // kmax, phase_slots, plan_words when u8
 };
 
 struct Segment {  // trials [t0, t0 + nt) (sorted order) of one observation
@@ -637,7 +637,8 @@ int psr_batch_create(psr_ctx* c, psr_obs* const* obs, size_t nobs, const double*
         o.u8 = false;
         if (o.obs->kind == KIND_U8) {
             o.spread = spread_bounds(o.obs, o.sorted.data(), o.ntrials, DEDISP_BLOCK_TRIALS);
-            o.u8 = dedisp_job_setup(o.obs->nchans, o.obs->maxval, o.obs->pitch, o.spread.data(), &o.proto);
+            o.u8 = dedisp_job_setup(o.obs->nchans, o.obs->maxval, o.obs->pitch, o.spread.data(), (uint32_t)o.n,
+                                    max_delay_at(o.obs, o.sorted[o.ntrials - 1]), &o.proto);
         }
     }
     for (Class& cl : b->classes) {

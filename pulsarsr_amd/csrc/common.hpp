@@ -83,16 +83,21 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // integer dedispersion (u8 rows), outputs f32 exact sums or FP16 scaled
 // DM trials per dedispersion workgroup (the delay spread bound is per block)
 constexpr uint32_t DEDISP_BLOCK_TRIALS = 64;
+// Bytes allocated after the last channel row of every u8 observation: the
+// LDS-staged dedispersion reads a channel row a few hundred samples past the
+// last delayed sample it sums.
+constexpr uint64_t DD_ALLOC_SLACK = 4096;
 // spread[c] (host, per channel) bounds the delay spread of channel c over any
-// block of DEDISP_BLOCK_TRIALS trials.  Returns false (and launches nothing)
-// when the spreads need more LDS than a workgroup has or the totals could pass
-// 2^20; the caller then uses launch_dedisp_f32.  `table` holds
+// block of DEDISP_BLOCK_TRIALS trials; max_delay bounds every delay of the
+// launch.  Returns false (and launches nothing) when the spreads need more LDS
+// than a workgroup has, the totals could pass 2^20 or the reads could pass the
+// allocation; the caller then uses launch_dedisp_f32.  `table` holds
 // dedisp_u8_table_words(nchans, ntrials) words: the blocks' channel bases and
 // the per-block plans (stages of channel-pair rows) a planner kernel writes.
 bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
                       uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
                       uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
-                      double* partial, const uint32_t* spread);
+                      double* partial, const uint32_t* spread, uint64_t max_delay);
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials);
 
 // One observation's share (a range of its DM trials) of a many-observation
@@ -111,17 +116,18 @@ struct DedispJob {
     void* out;             // series row of the job's first trial
     double* partial;       // [ntrials][ntiles]
     uint32_t nchans, ntrials, nblk, ntiles, n, out_pitch;
-    uint32_t phase_slots, RB, kmax, windows;
+    uint32_t phase_slots, kmax;
     float m0, inv_s0;
     uint32_t entry0, blk0, item0, trial0;  // first delay entry, plan block, work item, trial in the batch
     double m0d, inv_s0d;                   // FP16 storage offset and 1/scale in f64 (trial means)
 };
-// Fills RB, windows, kmax, phase_slots and plan_words for a job with nchans
-// u8 rows of `pitch` bytes, samples <= maxval and per-channel delay spreads
-// `spread` over any 64-trial block; false when the LDS-staged kernel cannot
-// take it (the caller then dedisperses it with launch_dedisp_f32).
-bool dedisp_job_setup(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uint32_t* spread,
-                      DedispJob* j);
+// Fills kmax, phase_slots and plan_words for a job with nchans u8 rows of
+// `pitch` bytes, samples <= maxval, per-channel delay spreads `spread` over any
+// 64-trial block, series length n and delays <= max_delay; false when the
+// LDS-staged kernel cannot take it (the caller then dedisperses it with
+// launch_dedisp_f32).
+bool dedisp_job_setup(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uint32_t* spread, uint32_t n,
+                      uint64_t max_delay, DedispJob* j);
 uint32_t dedisp_tiles(uint32_t n);  // 512-output time tiles of a series
 size_t dedisp_plan_words(size_t nchans);  // plan words per 64-trial block
 // delays, plans and the persistent work-queue dedispersion of a batch of jobs:

@@ -80,6 +80,9 @@ int resolve_length(const psr_obs* o, const double* dms, size_t ntrials, size_t l
 // per-channel delay-spread bound over any `block` consecutive sorted trials
 std::vector<uint32_t> spread_bounds(const psr_obs* o, const double* dms_sorted, size_t ntrials,
                                     size_t block);
+// the largest channel delay at DM dm (delays grow with the DM: the last of a
+// sorted list bounds them all)
+uint64_t max_delay_at(const psr_obs* o, double dm);
 // search::periodicity's bin limits and birdie bins for a geometry
 int make_params(const SpecGeom& g, double tsamp, const psr_search_cfg* cfg, SearchParams* p,
                 std::vector<uint32_t>* birdie_bins);

@@ -166,18 +166,19 @@ void launch_delays(hipStream_t s, const double* inv_f2, uint32_t nchans, double 
 // are exact.
 //
 // A planner kernel lays out each block as stages: slots (a slot is a channel
-// pair or a single channel) whose rows fit the LDS row area and whose raw
-// windows (one per channel) fit the raw area, taken from both ends of the band
-// in turn so that every stage mixes wide and narrow rows.  A stage is one
-// fixed-size record -- header, window channels and offsets, row records, every
-// trial's 16-bit LDS read address per slot and the row of every quad (four row
-// words).  Staging is pipelined: while a stage is summed, the raw windows of
-// the next stage and the record of the stage after it are loaded straight
-// into LDS (MUBUF loads with the LDS bit, no registers held, and no scalar
-// loads, whose out-of-order returns would force full LDS waits in the sums);
-// at the stage boundary all 512 lanes turn the raw bytes into the rows, one
-// quad per lane at a time.  Two workgroups share a CU, so one's row build
-// overlaps the other's sums.
+// pair or a single channel) whose rows fit the LDS row area, taken from both
+// ends of the band in turn so that every stage mixes wide and narrow rows.  A
+// stage is one fixed-size record -- header, row records, every trial's 16-bit
+// LDS read address per slot and the row of every quad (four row words).  At a
+// stage boundary all 512 lanes build the stage's rows, one quad per lane at a
+// time, straight from the channel rows in global memory (L2-resident: the
+// workgroups of one XCD work on the same time tile): a quad's eight dwords of
+// channel c are aligned, those of channel c + 1 are unaligned dword loads at
+// the pair's relative delay, so the build costs the LDS only its row stores.
+// The record of the next stage is loaded into registers during the sums and
+// stored at the next boundary (no scalar loads, whose out-of-order returns
+// would force full LDS waits in the sums).  Two workgroups share a CU, so
+// one's row build overlaps the other's sums.
 //
 // Workgroups are numbered XCD-aware: consecutive workgroup ids go to different
 // XCDs, so id is remapped so that each XCD walks a contiguous run of
@@ -191,24 +192,24 @@ constexpr int DD_TRIALS = DD_WAVES * DD_M;
 static_assert(DD_TRIALS == (int)DEDISP_BLOCK_TRIALS, "block size shared with the host");
 constexpr int DD_XCDS = 8;
 constexpr int DD_AHEAD = 2;  // trials whose LDS reads are issued ahead of their adds
-constexpr int DD_ROW_BYTES = 48 * 1024;  // rows of one stage (at most; two workgroups per CU)
-constexpr int DD_RAW_BYTES = 20 * 1024;  // raw byte windows of one stage (at most)
-constexpr int DD_RSRC_WORD3 = 0x00020000;  // buffer descriptor word 3: raw 32-bit data
-constexpr int DD_RECS = 3;  // stage records held: summed, windows loading, record loading
-constexpr int DDP_MAX_WINDOWS = 20;
+constexpr int DD_ROW_BYTES = 64 * 1024;  // rows of one stage (at most; 16-bit LDS addresses)
+constexpr int DD_RECS = 2;  // stage records held: the one summed, the next one
+// Bytes readable past the last channel row of a u8 observation (its
+// allocation's slack): the row build reads up to sample t0 + delay + 519 of a
+// channel, which the host bounds by the row pitch plus this slack.
+constexpr uint64_t DD_READ_REACH = 520;
 constexpr int DDP_MAX_ROWS = 24;                  // rows per stage
 constexpr int DDP_MAX_SLOTS = 16;                 // slots per stage
 constexpr int DDP_KMAX = 3;                       // pair rows per channel pair (at most)
-// stage record (u32 words): [0] nwin | nslot << 8 | nrow << 16, [1] quads
-// (4-word units of the row buffer), [4 ..] window channels, [DDP_WIN_START ..]
-// window byte offsets from the tile start, [DDP_ROW_REC + 4 r ..] row r:
-// rowoff | quads << 16, raw byte of word L of channel c in its window, the same
-// for channel c + 1, cwin | c1win << 8 | paired << 16; [DDP_LAD_REC + 32 s ..]
-// read addresses of slot s (DD_TRIALS x u16); [DDP_QMAP_REC ..] the row of
-// every quad (u8).  Rows are packed in record order, so quad q of the stage is
-// quad q - rowoff / 64 of its row and lands at byte 64 q of the row buffer.
-constexpr int DDP_WIN_START = 4 + DDP_MAX_WINDOWS;
-constexpr int DDP_ROW_REC = DDP_WIN_START + DDP_MAX_WINDOWS;
+// stage record (u32 words): [0] nslot << 8 | nrow << 16, [1] quads (4-word
+// units of the row buffer), [DDP_ROW_REC + 4 r ..] row r: rowoff | quads << 16,
+// channel c | paired << 16, the sample of word 0 lane 0 of channel c relative
+// to the tile start, the same for channel c + 1 (paired rows);
+// [DDP_LAD_REC + 32 s ..] read addresses of slot s (DD_TRIALS x u16);
+// [DDP_QMAP_REC ..] the row of every quad (u8).  Rows are packed in record
+// order, so quad q of the stage is quad q - rowoff / 64 of its row and lands at
+// byte 64 q of the row buffer.
+constexpr int DDP_ROW_REC = 4;
 constexpr int DDP_LAD_REC = DDP_ROW_REC + 4 * DDP_MAX_ROWS;
 constexpr int DDP_MAX_QUADS = DD_ROW_BYTES / 64;  // 4 row words of 16 bytes
 constexpr int DDP_QMAP_REC = DDP_LAD_REC + DDP_MAX_SLOTS * DD_TRIALS / 2;
@@ -226,19 +227,12 @@ static size_t ddp_plan_words(uint32_t nchans) {  // per block; stages <= slots <
     return DDP_PLAN_HEAD + (size_t)nchans * DDP_STAGE_PITCH;
 }
 
-// Raw window of channel ch in a tile starting at t0: bytes from
-// t0 + (base & ~15) - 16, so sample t0 + base + i sits at byte mis + i with
-// mis = (base & 15) + 16 (at least 13 bytes of margin below any word a pair
-// row of channel c + 1 reads).
-__device__ __forceinline__ uint32_t ddp_mis(int32_t base) { return (uint32_t)(base & 15) + 16u; }
-
 // One workgroup per block of DD_TRIALS trials.  Dynamic LDS: bases[nchans],
 // per slot its stage | position << 16, then per channel pair its kind (K pair
 // rows, 0 = single-channel rows) and first slot, and per row its delta,
 // L | W << 16 and LDS offset.
 __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays, uint32_t nchans,
-                                              uint32_t ntrials, uint32_t kmax, uint32_t max_windows,
-                                              uint32_t row_budget, int32_t* __restrict__ bases,
+                                              uint32_t ntrials, uint32_t kmax, uint32_t row_budget, int32_t* __restrict__ bases,
                                               uint32_t* __restrict__ plan, uint64_t plan_words,
                                               uint32_t blk) {
     extern __shared__ int32_t sbase[];
@@ -326,25 +320,26 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
     // integer sums are exact in any channel order).
     if (threadIdx.x == 0) {
         uint32_t nst = 0, nslot_all = 0;
-        uint32_t nwin = 0, nslot = 0, nrow = 0, bytes = 0;
+        uint32_t nslot = 0, nrow = 0, bytes = 0;
         uint32_t* st = pb + DDP_PLAN_HEAD;
         auto close = [&]() {
             if (!nslot) return;
-            st[0] = nwin | nslot << 8 | nrow << 16;
+            st[0] = nslot << 8 | nrow << 16;
             st[1] = bytes / 64;
             nst++;
             st = pb + DDP_PLAN_HEAD + (size_t)nst * DDP_STAGE_PITCH;
-            nwin = nslot = nrow = bytes = 0;
+            nslot = nrow = bytes = 0;
         };
-        auto put_row = [&](uint32_t ui, uint32_t W, uint32_t offc, uint32_t offc1, uint32_t cw, uint32_t c1w,
-                           uint32_t paired) {
+        // row record: word 0 lane 0 of the row is sample off_c of channel ch
+        // (and off_c1 of channel ch + 1 for a pair row), relative to the tile
+        auto put_row = [&](uint32_t ui, uint32_t W, uint32_t ch, int32_t off_c, int32_t off_c1, uint32_t paired) {
             uRO[ui] = bytes;
             uRN[ui] = (uint8_t)nrow;
             uint32_t* r = st + DDP_ROW_REC + 4 * nrow;
             r[0] = bytes | (W / 4) << 16;
-            r[1] = offc;
-            r[2] = offc1;
-            r[3] = cw | c1w << 8 | paired << 16;
+            r[1] = ch | paired << 16;
+            r[2] = (uint32_t)off_c;
+            r[3] = (uint32_t)off_c1;
             bytes += 16 * W;
             nrow++;
         };
@@ -353,9 +348,8 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
             nslot++;
             nslot_all++;
         };
-        auto fits = [&](uint32_t w, uint32_t r, uint32_t b) {
-            return nwin + w <= max_windows && nrow + r <= (uint32_t)DDP_MAX_ROWS && bytes + b <= row_budget &&
-                   nslot < (uint32_t)DDP_MAX_SLOTS;
+        auto fits = [&](uint32_t r, uint32_t b) {
+            return nrow + r <= (uint32_t)DDP_MAX_ROWS && bytes + b <= row_budget && nslot < (uint32_t)DDP_MAX_SLOTS;
         };
         for (uint32_t i = 0; i < npair; i++) {
             const uint32_t u = i & 1 ? npair - 1 - i / 2 : i / 2;
@@ -372,30 +366,22 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
                 uint32_t need = 0;
 #pragma unroll
                 for (int k = 0; k < DDP_KMAX; k++) need += (uint32_t)k < K ? 16 * (LW[k] >> 16) : 0u;
-                if (!fits(2, K, need)) close();
+                if (!fits(K, need)) close();
                 uslot[u] = nslot_all;
-                st[4 + nwin] = c;
-                st[5 + nwin] = c + 1;
-                st[DDP_WIN_START + nwin] = (uint32_t)((sb0 & ~15) - 16);
-                st[DDP_WIN_START + 1 + nwin] = (uint32_t)((sb1 & ~15) - 16);
 #pragma unroll
                 for (int k = 0; k < DDP_KMAX; k++) {
                     if ((uint32_t)k >= K) break;
                     const uint32_t L = LW[k] & 0xffffu, W = LW[k] >> 16;
-                    const int32_t o1 = (int32_t)ddp_mis(sb1) + (int32_t)L + dl[k] - (sb1 - sb0);
-                    put_row(u * DDP_KMAX + k, W, ddp_mis(sb0) + L, (uint32_t)o1, nwin, nwin + 1, 1);
+                    put_row(u * DDP_KMAX + k, W, c, sb0 + (int32_t)L, sb0 + (int32_t)L + dl[k], 1);
                 }
-                nwin += 2;
                 put_slot();
             } else {
                 for (uint32_t j = 0; j < 2 && c + j < nchans; j++) {
                     const uint32_t L = LW[j] & 0xffffu, W = LW[j] >> 16;
-                    if (!fits(1, 1, 16 * W)) close();
+                    if (!fits(1, 16 * W)) close();
                     if (j == 0) uslot[u] = nslot_all;
-                    st[4 + nwin] = c + j;
-                    st[DDP_WIN_START + nwin] = (uint32_t)(((j ? sb1 : sb0) & ~15) - 16);
-                    put_row(u * DDP_KMAX + j, W, ddp_mis(j ? sb1 : sb0) + L, 0, nwin, 0, 0);
-                    nwin++;
+                    const int32_t sb = j ? sb1 : sb0;
+                    put_row(u * DDP_KMAX + j, W, c + j, sb + (int32_t)L, sb + (int32_t)L, 0);
                     put_slot();
                 }
             }
@@ -458,51 +444,86 @@ __device__ __forceinline__ void dd_plan_block(const int32_t* __restrict__ delays
 
 
 __global__ __launch_bounds__(256) void k_dd_plan(const int32_t* __restrict__ delays, uint32_t nchans,
-                                                 uint32_t ntrials, uint32_t kmax, uint32_t max_windows,
-                                                 uint32_t row_budget, int32_t* __restrict__ bases,
-                                                 uint32_t* __restrict__ plan, uint64_t plan_words) {
-    dd_plan_block(delays, nchans, ntrials, kmax, max_windows, row_budget, bases, plan, plan_words,
-                  blockIdx.x);
+                                                 uint32_t ntrials, uint32_t kmax, uint32_t row_budget,
+                                                 int32_t* __restrict__ bases, uint32_t* __restrict__ plan,
+                                                 uint64_t plan_words) {
+    dd_plan_block(delays, nchans, ntrials, kmax, row_budget, bases, plan, plan_words, blockIdx.x);
 }
 
-// One work item of the u8 dedispersion: DM block `blk` (trials blk * 64 ..) over
-// time tile `tile` (outputs tile * 512 ..).  plan is the whole launch's table
-// (indexed by blk), partial has ntiles words per trial.
-__device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64_t pitch,
-                                        uint32_t phase_slots, const uint32_t* __restrict__ plan,
-                                        uint64_t plan_words, uint32_t ntrials, uint32_t ntiles, uint32_t n,
-                                        void* __restrict__ out, uint32_t out_pitch, int fp16, float m0,
-                                        float inv_s0, double* __restrict__ partial, uint32_t RB,
-                                        uint32_t tile, uint32_t blk) {
-    // LDS: the raw windows [window][RB], the stage's rows and three stage
-    // records (the current one, the next, whose raw windows are loaded during
-    // this stage, and the one after).  Windows arrive by direct global->LDS
-    // loads into their own LDS object, which the sums never read, so no read
-    // of the sums waits for them; records go through registers (loaded during
-    // one stage's sums, stored at the next stage boundary) for the same
-    // reason: the sums read their LDS addresses from the records.
-    __shared__ __align__(16) uint8_t raw[DD_RAW_BYTES];
-    __shared__ __align__(16) uint32_t lds[DD_ROW_BYTES / 4];
-    __shared__ __align__(16) uint32_t srec[DD_RECS][DDP_STAGE_PITCH];
+// Record of stage st, moved through registers: thread t of nt (nt >= half a
+// record's dwords) holds dwords 2 t and 2 t + 1 (indices clamped into range;
+// the surplus is never stored).
+__device__ __forceinline__ uint2 dd_rec_load(const uint32_t* __restrict__ pb, uint32_t st, uint32_t t) {
+    const uint32_t* src = pb + DDP_PLAN_HEAD + (size_t)st * DDP_STAGE_PITCH;
+    const uint32_t i = min(2 * t, (uint32_t)DDP_STAGE_PITCH - 2);
+    return make_uint2(src[i], src[i + 1]);
+}
+__device__ __forceinline__ void dd_rec_store(uint32_t* slot, uint2 v, uint32_t t) {
+    if (2 * t < (uint32_t)DDP_STAGE_PITCH) *(uint2*)(slot + 2 * t) = v;
+}
+static_assert(2 * DD_THREADS >= DDP_STAGE_PITCH, "a record is two dwords per thread");
 
-    const uint32_t t0 = tile * DD_TILE;
-    const uint32_t d0 = blk * DD_TRIALS;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t* pb = plan + (size_t)blk * plan_words;
-    const uint32_t nstages = __builtin_amdgcn_readfirstlane(pb[0]);
-
-    // acc16: this phase's sums (u16 lanes, none can overflow within a phase);
-    // lo: running sums mod 2^16 per lane; cy: carries out of each lane, 4 bits
-    // per output (bits 4j + 16h for dword j, half h; the host keeps every total
-    // below 2^20)
-    uint32_t acc16[DD_M][4], lo[DD_M][4], cy[DD_M];
+// The rows of the stage whose record is sr, built into `rows` by threads
+// t = 0 .. nt - 1: the threads take the stage's quads (4 row words) in turn;
+// quad q belongs to row qmap[q] and lands at row word 16 q.  A thread builds
+// its quad's words from eight dwords of channel c (samples off_c + 4 g + 64 k,
+// k < 8, g = the quad's index in its row: aligned) and eight of channel c + 1
+// (samples off_c1 + 4 g + 64 k: unaligned dword loads), read from the channel
+// rows in global memory (L2-resident: the workgroups of one XCD work on the
+// same time tile), and stores them in a lane-rotated order, which spreads
+// every 16 lanes' stores over all LDS banks.  Single-channel rows read their
+// own channel twice and zero the second copy's bytes with an all-zero byte
+// select.
+__device__ __forceinline__ void dd_build(const uint32_t* sr, uint32_t* rows, const uint8_t* __restrict__ tile_base,
+                                         uint64_t pitch, uint32_t t, uint32_t nt) {
+    const uint32_t nq = __builtin_amdgcn_readfirstlane(sr[1]);
+    const uint8_t* qmap = (const uint8_t*)(sr + DDP_QMAP_REC);
+    for (uint32_t q = t; q < nq; q += nt) {
+        const uint32_t r = qmap[q];
+        const uint4 rec = *(const uint4*)(sr + DDP_ROW_REC + 4 * r);
+        const uint32_t g = q - (rec.x & 0xffffu) / 64;
+        const bool paired = (rec.y >> 16) & 1u;
+        const uint8_t* pc = tile_base + (size_t)(rec.y & 0xffffu) * pitch + (int32_t)(rec.z + 4 * g);
+        const uint8_t* p1 = paired ? pc + pitch + ((int32_t)rec.w - (int32_t)rec.z) : pc;
+        uint32_t d[8], e[8];
 #pragma unroll
-    for (int m = 0; m < DD_M; m++) {
+        for (int k = 0; k < 8; k++) {
+            d[k] = *(const uint32_t*)(pc + 64 * k);
+            __builtin_memcpy(&e[k], p1 + 64 * k, 4);
+        }
+        const uint32_t rot = (q >> 2) & 3u;
+        uint32_t* row = rows + 16 * q;
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc16[m][j] = lo[m][j] = 0;
-        cy[m] = 0;
+        for (uint32_t bb = 0; bb < 4; bb++) {
+            const uint32_t b = (bb + rot) & 3u;
+            const uint32_t sd = 0x0c040c00u + 0x00010001u * b, se = paired ? sd : 0x0c0c0c0cu;
+            *(uint4*)(row + 4 * b) = make_uint4(
+                __builtin_amdgcn_perm(d[1], d[0], sd) + __builtin_amdgcn_perm(e[1], e[0], se),
+                __builtin_amdgcn_perm(d[3], d[2], sd) + __builtin_amdgcn_perm(e[3], e[2], se),
+                __builtin_amdgcn_perm(d[5], d[4], sd) + __builtin_amdgcn_perm(e[5], e[4], se),
+                __builtin_amdgcn_perm(d[7], d[6], sd) + __builtin_amdgcn_perm(e[7], e[6], se));
+        }
     }
-    auto widen = [&]() {
+}
+
+// One summing wave's 8 trials over a 512-output tile.  acc16: this phase's
+// sums (u16 lanes, none can overflow within a phase); lo: running sums mod
+// 2^16 per lane; cy: carries out of each lane, 4 bits per output (bits
+// 4j + 16h for dword j, half h; the host keeps every total below 2^20).
+struct DdSums {
+    uint32_t acc16[DD_M][4], lo[DD_M][4], cy[DD_M];
+    uint32_t in_phase;
+
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int m = 0; m < DD_M; m++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc16[m][j] = lo[m][j] = 0;
+            cy[m] = 0;
+        }
+        in_phase = 0;
+    }
+    __device__ __forceinline__ void widen() {
 #pragma unroll
         for (int m = 0; m < DD_M; m++)
 #pragma unroll
@@ -514,115 +535,20 @@ __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64
                 cy[m] += ((x >> 16) << (4 * j)) + ((y >> 16) << (4 * j + 16));
                 acc16[m][j] = 0;
             }
-    };
-    uint32_t in_phase = 0;
-    const char* lds_lane = (const char*)lds + 16 * (threadIdx.x & 63);
-    // record of stage st: thread t holds dwords 2 t and 2 t + 1 (indices
-    // clamped into range; the surplus is never stored)
-    auto rec_load = [&](uint32_t st) {
-        const uint32_t* src = pb + DDP_PLAN_HEAD + (size_t)st * DDP_STAGE_PITCH;
-        const uint32_t i = min(2 * (uint32_t)threadIdx.x, (uint32_t)DDP_STAGE_PITCH - 2);
-        return make_uint2(src[i], src[i + 1]);
-    };
-    auto rec_store = [&](uint32_t st, uint2 v) {
-        if (2 * threadIdx.x < (uint32_t)DDP_STAGE_PITCH) *(uint2*)(srec[st % DD_RECS] + 2 * threadIdx.x) = v;
-    };
-    static_assert(2 * DD_THREADS >= DDP_STAGE_PITCH, "a record is two dwords per thread");
-    // raw windows of stage st: wave w loads windows w, w + 8 and w + 16 (lanes
-    // whose 16 bytes fall outside the row read its nearest 16 in-row bytes
-    // instead: those samples only reach outputs at or past n, or words no
-    // trial reads)
-    auto load_windows = [&](uint32_t st) {
-        const uint32_t* sr = srec[st % DD_RECS];
-        const uint32_t nwin = __builtin_amdgcn_readfirstlane(sr[0]) & 0xffu;
-        uint32_t ln = threadIdx.x;
-        asm volatile("" : "+v"(ln));
-        ln &= 63;
-        for (uint32_t wi = (uint32_t)wave; wi < nwin; wi += DD_WAVES) {
-            const uint32_t c = __builtin_amdgcn_readfirstlane(sr[4 + wi]);
-            // 32-bit offsets: t0 + start < pitch < 2^31
-            const int32_t start = (int32_t)t0 + (int32_t)__builtin_amdgcn_readfirstlane(sr[DDP_WIN_START + wi]);
-            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
-                (void*)(data + (size_t)c * pitch), 0, (int)pitch, DD_RSRC_WORD3);
-            for (uint32_t k = 0; k < RB; k += 1024) {
-                int32_t p = start + (int32_t)(k + 16 * ln);
-                p = p < 0 ? 0 : (p > (int32_t)pitch - 16 ? (int32_t)pitch - 16 : p);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, raw + (size_t)wi * RB + k, 16, (uint32_t)p, 0, 0, 0);
-            }
-        }
-    };
-    // rows of stage st: the 512 lanes take the stage's quads (4 row words) in
-    // turn; quad q belongs to row qmap[q] and lands at row-buffer word 16 q.  A
-    // lane builds its quad's words from dword g + 16 k (k < 8) of the windows
-    // (g = the quad's index in its row) and stores them in a lane-rotated
-    // order, which spreads every 16 lanes' stores over all LDS banks.
-    // Single-channel rows read their own window twice and zero the second
-    // channel's bytes with an all-zero byte select.
-    auto build = [&](uint32_t st) {
-        const uint32_t* sr = srec[st % DD_RECS];
-        const uint32_t nq = __builtin_amdgcn_readfirstlane(sr[1]);
-        const uint8_t* qmap = (const uint8_t*)(sr + DDP_QMAP_REC);
-        for (uint32_t q = threadIdx.x; q < nq; q += DD_THREADS) {
-            const uint32_t r = qmap[q];
-            const uint4 rec = *(const uint4*)(sr + DDP_ROW_REC + 4 * r);
-            const uint32_t g = q - (rec.x & 0xffffu) / 64;
-            const bool paired = (rec.w >> 16) & 1u;
-            const uint32_t* rc = (const uint32_t*)(raw + (rec.w & 0xffu) * RB + rec.y) + g;
-            const uint32_t* r1 =
-                paired ? (const uint32_t*)(raw + ((rec.w >> 8) & 0xffu) * RB + (rec.z & ~3u)) + g : rc;
-            const uint32_t sh = paired ? rec.z & 3u : 0u;
-            uint32_t d[8], e[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                d[k] = rc[16 * k];
-                e[k] = __builtin_amdgcn_alignbyte(r1[16 * k + 1], r1[16 * k], sh);
-            }
-            const uint32_t rot = (q >> 2) & 3u;
-            uint32_t* row = lds + 16 * q;
-#pragma unroll
-            for (uint32_t bb = 0; bb < 4; bb++) {
-                const uint32_t b = (bb + rot) & 3u;
-                const uint32_t sd = 0x0c040c00u + 0x00010001u * b, se = paired ? sd : 0x0c0c0c0cu;
-                *(uint4*)(row + 4 * b) = make_uint4(
-                    __builtin_amdgcn_perm(d[1], d[0], sd) + __builtin_amdgcn_perm(e[1], e[0], se),
-                    __builtin_amdgcn_perm(d[3], d[2], sd) + __builtin_amdgcn_perm(e[3], e[2], se),
-                    __builtin_amdgcn_perm(d[5], d[4], sd) + __builtin_amdgcn_perm(e[5], e[4], se),
-                    __builtin_amdgcn_perm(d[7], d[6], sd) + __builtin_amdgcn_perm(e[7], e[6], se));
-            }
-        }
-    };
-    {
-        const uint2 r0 = rec_load(0), r1 = rec_load(min(1u, nstages - 1));  // nstages >= 1: nchans >= 1
-        rec_store(0, r0);
-        if (nstages > 1) rec_store(1, r1);
+        in_phase = 0;
     }
-    __syncthreads();
-    load_windows(0);
-    uint2 rnext = make_uint2(0u, 0u);  // record of stage st + 1, loaded during stage st - 1
-    for (uint32_t st = 0; st < nstages; st++) {
-        const uint32_t* sr = srec[st % DD_RECS];
+    // The slots of the stage whose record is sr, read from rows at lds_lane
+    // (this lane's 16 bytes of the row buffer), in pairs: one v_add3_u32 adds
+    // both slots' u16 lanes (no lane can carry within a phase: every slot adds
+    // at most 2 * maxval per lane).  The wave's 8 read addresses per slot come
+    // from 16-byte broadcast reads (16 bits each), fetched one pair ahead.
+    __device__ __forceinline__ void stage(const uint32_t* sr, const char* lds_lane, int wave,
+                                          uint32_t phase_slots) {
         const uint32_t nslot = (__builtin_amdgcn_readfirstlane(sr[0]) >> 8) & 0xffu;
-        // this wave's loads (stage st's windows, stage st + 1's record) have
-        // landed; after the barrier every wave's have, and the previous stage's
-        // sums are done with the rows and with stage st - 1's record slot
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-        __syncthreads();
-        if (st >= 1 && st + 1 < nstages) rec_store(st + 1, rnext);
-        build(st);
-        __syncthreads();
-        if (st + 1 < nstages) load_windows(st + 1);
-        rnext = rec_load(min(st + 2, nstages - 1));
-        // slots in pairs: one v_add3_u32 adds both slots' u16 lanes (no lane can
-        // carry within a phase: every slot adds at most 2 * maxval per lane).
-        // The wave's 8 read addresses per slot come from 16-byte broadcast reads
-        // (16 bits each), fetched one pair ahead.
         const uint16_t* lw = (const uint16_t*)(sr + DDP_LAD_REC) + wave * DD_M;
         uint4 A = *(const uint4*)(lw), B = *(const uint4*)(lw + DD_TRIALS);
         for (uint32_t c = 0; c + 1 < nslot; c += 2) {
-            if (in_phase + 2 > phase_slots) {
-                widen();
-                in_phase = 0;
-            }
+            if (in_phase + 2 > phase_slots) widen();
             const uint32_t pa[4] = {A.x, A.y, A.z, A.w}, pb2[4] = {B.x, B.y, B.z, B.w};
             uint4 x[DD_M], y[DD_M];
 #pragma unroll
@@ -662,10 +588,7 @@ __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64
             in_phase += 2;
         }
         if (nslot & 1) {  // odd slot count: the stage's last slot alone
-            if (in_phase + 1 > phase_slots) {
-                widen();
-                in_phase = 0;
-            }
+            if (in_phase + 1 > phase_slots) widen();
             const uint4 A2 = *(const uint4*)(lw + (nslot - 1) * DD_TRIALS);
             const uint32_t pa[4] = {A2.x, A2.y, A2.z, A2.w};
 #pragma unroll
@@ -680,64 +603,109 @@ __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64
             in_phase += 1;
         }
     }
-    widen();
-    // totals: low 16 bits from lo, the carries above them; acc32[m][k] is output
-    // t0 + lane + 64 k (dword j holds k = 2j low, 2j + 1 high)
-    uint32_t acc32[DD_M][8];
+    // Totals (low 16 bits from lo, the carries above them) of trials
+    // d0 + wave * 8 + m, output t0 + lane + 64 k, written as f32 or as the FP16
+    // storage value, and each trial's tile sum (exact in f64) into partial.
+    __device__ __forceinline__ void finish(uint32_t d0, int wave, uint32_t t0, uint32_t tile, uint32_t ntrials,
+                                           uint32_t ntiles, uint32_t n, void* __restrict__ out, uint32_t out_pitch,
+                                           int fp16, float m0, float inv_s0, double* __restrict__ partial) {
+        widen();
+        uint32_t acc32[DD_M][8];  // acc32[m][k]: dword j holds k = 2j low, 2j + 1 high
 #pragma unroll
-    for (int m = 0; m < DD_M; m++)
+        for (int m = 0; m < DD_M; m++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            acc32[m][2 * j] = (lo[m][j] & 0xffffu) + (((cy[m] >> (4 * j)) & 15u) << 16);
-            acc32[m][2 * j + 1] = (lo[m][j] >> 16) + (((cy[m] >> (4 * j + 16)) & 15u) << 16);
-        }
-    const int m0i = (int)m0;
-    uint32_t ln = threadIdx.x;
-    asm volatile("" : "+v"(ln));
-    ln &= 63;
+            for (int j = 0; j < 4; j++) {
+                acc32[m][2 * j] = (lo[m][j] & 0xffffu) + (((cy[m] >> (4 * j)) & 15u) << 16);
+                acc32[m][2 * j + 1] = (lo[m][j] >> 16) + (((cy[m] >> (4 * j + 16)) & 15u) << 16);
+            }
+        const int m0i = (int)m0;
+        uint32_t ln = threadIdx.x;
+        asm volatile("" : "+v"(ln));
+        ln &= 63;
 #pragma unroll
-    for (int m = 0; m < DD_M; m++) {
-        const uint32_t d = d0 + wave * DD_M + m;
-        double lsum = 0.0;
-        if (d < ntrials) {
+        for (int m = 0; m < DD_M; m++) {
+            const uint32_t d = d0 + wave * DD_M + m;
+            double lsum = 0.0;
+            if (d < ntrials) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t t = t0 + ln + 64 * k;
-                if (t < n) {
-                    const uint32_t v = acc32[m][k];
-                    const size_t o = (size_t)d * out_pitch + t;
-                    lsum += (double)v;
-                    if (fp16)
-                        ((__half*)out)[o] = __float2half_rn((float)((int)v - m0i) * inv_s0);
-                    else
-                        ((float*)out)[o] = (float)v;
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t t = t0 + ln + 64 * k;
+                    if (t < n) {
+                        const uint32_t v = acc32[m][k];
+                        const size_t o = (size_t)d * out_pitch + t;
+                        lsum += (double)v;
+                        if (fp16)
+                            ((__half*)out)[o] = __float2half_rn((float)((int)v - m0i) * inv_s0);
+                        else
+                            ((float*)out)[o] = (float)v;
+                    }
                 }
             }
-        }
-        // wave reduction of the tile's sum (integers < 2^53: exact in any order)
+            // wave reduction of the tile's sum (integers < 2^53: exact in any order)
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
-        if (ln == 0 && d < ntrials) partial[(size_t)d * ntiles + tile] = lsum;
+            for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+            if (ln == 0 && d < ntrials) partial[(size_t)d * ntiles + tile] = lsum;
+        }
     }
+};
+
+// One work item of the u8 dedispersion: DM block `blk` (trials blk * 64 ..) over
+// time tile `tile` (outputs tile * 512 ..).  plan is the whole launch's table
+// (indexed by blk), partial has ntiles words per trial.  Eight waves (two
+// workgroups per CU): at each stage boundary all of them build the stage's
+// rows, then each sums its 8 trials; the other workgroup of the CU overlaps.
+__device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64_t pitch,
+                                        uint32_t phase_slots, const uint32_t* __restrict__ plan,
+                                        uint64_t plan_words, uint32_t ntrials, uint32_t ntiles, uint32_t n,
+                                        void* __restrict__ out, uint32_t out_pitch, int fp16, float m0,
+                                        float inv_s0, double* __restrict__ partial, uint32_t tile,
+                                        uint32_t blk) {
+    // LDS: the stage's rows and two stage records (the one summed and the
+    // next).  Records go through registers (loaded during one stage's sums,
+    // stored at the next stage boundary): the sums read their LDS addresses
+    // from the records, and a direct global->LDS load would make the sums'
+    // first LDS read wait for it.
+    __shared__ __align__(16) uint32_t lds[DD_ROW_BYTES / 4];
+    __shared__ __align__(16) uint32_t srec[DD_RECS][DDP_STAGE_PITCH];
+    const uint32_t t0 = tile * DD_TILE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t* pb = plan + (size_t)blk * plan_words;
+    const uint32_t nstages = __builtin_amdgcn_readfirstlane(pb[0]);
+    const char* lds_lane = (const char*)lds + 16 * (threadIdx.x & 63);
+    DdSums sums;
+    sums.init();
+    dd_rec_store(srec[0], dd_rec_load(pb, 0, threadIdx.x), threadIdx.x);  // nstages >= 1: nchans >= 1
+    uint2 rnext = dd_rec_load(pb, min(1u, nstages - 1), threadIdx.x);  // record st + 1, stored at stage st
+    for (uint32_t st = 0; st < nstages; st++) {
+        const uint32_t* sr = srec[st % DD_RECS];
+        // this wave's record load has landed; after the barrier every wave's
+        // has, record st is visible, and the previous stage's sums are done with
+        // the rows and with record st - 1's slot, which record st + 1 takes
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __syncthreads();
+        if (st + 1 < nstages) dd_rec_store(srec[(st + 1) % DD_RECS], rnext, threadIdx.x);
+        dd_build(sr, lds, data + t0, pitch, threadIdx.x, DD_THREADS);
+        __syncthreads();
+        rnext = dd_rec_load(pb, min(st + 2, nstages - 1), threadIdx.x);
+        sums.stage(sr, lds_lane, wave, phase_slots);
+    }
+    sums.finish(blk * DD_TRIALS, wave, t0, tile, ntrials, ntiles, n, out, out_pitch, fp16, m0, inv_s0, partial);
 }
 
 // One launch over the (tile, block) items of one observation; workgroup ids
 // are remapped XCD-aware (each XCD walks a contiguous run of items, block
 // fastest, so a tile's channel data is reused from that XCD's L2).
 __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_u8(
-    const uint8_t* __restrict__ data, uint64_t pitch, uint32_t nchans, uint32_t phase_slots,
-    const int32_t* __restrict__ bases, const uint32_t* __restrict__ plan, uint64_t plan_words,
-    uint32_t ntrials, uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out,
-    uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial, uint32_t RB) {
-    (void)nchans;
-    (void)bases;
+    const uint8_t* __restrict__ data, uint64_t pitch, uint32_t phase_slots, const uint32_t* __restrict__ plan,
+    uint64_t plan_words, uint32_t ntrials, uint32_t nblk, uint32_t ntiles, uint32_t n, void* __restrict__ out,
+    uint32_t out_pitch, int fp16, float m0, float inv_s0, double* __restrict__ partial) {
     const uint32_t total = ntiles * nblk;
     const uint32_t id = blockIdx.x;
     const uint32_t xcd = id % DD_XCDS, local = id / DD_XCDS;
     const uint32_t q = total / DD_XCDS, r = total % DD_XCDS;
     const uint32_t work = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + local;
-    dd_item(data, pitch, phase_slots, plan, plan_words, ntrials, ntiles, n, out, out_pitch, fp16, m0, inv_s0,
-            partial, RB, work / nblk, work % nblk);
+    dd_item(data, pitch, phase_slots, plan, plan_words, ntrials, ntiles, n, out, out_pitch, fp16, m0, inv_s0, partial,
+            work / nblk, work % nblk);
 }
 
 size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
@@ -745,30 +713,44 @@ size_t dedisp_u8_table_words(uint32_t nchans, uint32_t ntrials) {
     return nblk * nchans + nblk * ddp_plan_words(nchans);  // bases, then the plans
 }
 
-bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
-                      uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
-                      uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
-                      double* partial, const uint32_t* spread) {
+// Shape checks shared by the single- and many-observation launches: false
+// when the LDS-staged kernel cannot take the observation.  A row has
+// W <= spread + 3 + 64 + 3 words of 16 bytes; the row build reads a channel up
+// to sample t0 + delay + DD_READ_REACH (t0 <= n - 1), which must stay inside
+// the row pitch plus the slack after the last row.
+static bool dd_shape(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uint32_t* spread, uint32_t n,
+                     uint64_t max_delay, uint32_t* kmax_out, uint32_t* phase_out) {
     // 4-bit carry counters per output: every total must stay below 2^20;
-    // 16-byte raw loads; 32-bit buffer offsets; 16-bit stage numbers
-    if (nchans == 0 || ntrials == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
-    if (pitch % 16 || pitch < 16 || pitch >= (1ull << 31) || nchans > 0xffff) return false;
-    // A row has W <= spread + 3 + 64 + 3 words of 16 bytes; its raw windows are
-    // read up to byte spread + 560.
+    // aligned dword loads of channel rows; 16-bit channel numbers
+    if (nchans == 0 || n == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
+    if (pitch % 16 || pitch < 16 || nchans > 0xffff) return false;
+    if ((uint64_t)n - 1 + max_delay + DD_READ_REACH > pitch + DD_ALLOC_SLACK) return false;
     uint32_t smax = 0;
     for (uint32_t c = 0; c < nchans; c++) smax = std::max(smax, spread[c]);
     const uint64_t wmax = (uint64_t)smax + 6 + DD_TILE / 8 + 3;
-    const uint32_t RB = (uint32_t)(((uint64_t)smax + 600 + 1023) / 1024 * 1024);
-    if (RB > DD_RAW_BYTES) return false;
-    const uint32_t windows = std::min<uint32_t>(DDP_MAX_WINDOWS, DD_RAW_BYTES / RB);
-    if (windows < 2 || 16 * wmax > (uint64_t)DD_ROW_BYTES) return false;
+    if (16 * wmax > (uint64_t)DD_ROW_BYTES) return false;
     // a slot of kmax pair rows must fit the row area alone
     uint32_t kmax = DDP_KMAX;
     while (kmax > 1 && (uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax--;
     if ((uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax = 0;
+    if (ddp_plan_shm(nchans) > 160 * 1024) return false;
+    // slots per phase: no 16-bit lane can overflow (a slot adds at most 2 * maxval)
+    uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
+    phase = phase > 4096 ? 4096 : phase;
+    if (phase < 2) return false;
+    *kmax_out = kmax;
+    *phase_out = phase;
+    return true;
+}
+
+bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32_t nchans,
+                      uint32_t maxval, const int32_t* delays, int32_t* table, uint32_t ntrials,
+                      uint32_t n, void* out, uint32_t out_pitch, int fp16, float m0, float inv_s0,
+                      double* partial, const uint32_t* spread, uint64_t max_delay) {
+    uint32_t kmax = 0, phase = 0;
+    if (ntrials == 0 || !dd_shape(nchans, maxval, pitch, spread, n, max_delay, &kmax, &phase)) return false;
     const size_t nblk = (ntrials + DD_TRIALS - 1) / DD_TRIALS;
     const size_t plan_shm = ddp_plan_shm(nchans);
-    if (plan_shm > 160 * 1024) return false;
     int32_t* bases = table;
     uint32_t* plan = (uint32_t*)(table + nblk * nchans);
     const uint64_t pw = ddp_plan_words(nchans);
@@ -776,16 +758,11 @@ bool launch_dedisp_u8(hipStream_t s, const uint8_t* data, uint64_t pitch, uint32
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_dd_plan),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)plan_shm);
     hipLaunchKernelGGL(k_dd_plan, dim3((unsigned)nblk), dim3(256), plan_shm, s, delays, nchans, ntrials,
-                       kmax, windows, (uint32_t)DD_ROW_BYTES, bases, plan, pw);
-    // slots per phase: no 16-bit lane can overflow (a slot adds at most 2 * maxval)
-    uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
-    phase = phase > 4096 ? 4096 : phase;
-    if (phase < 2) return false;
+                       kmax, (uint32_t)DD_ROW_BYTES, bases, plan, pw);
     const uint32_t ntiles = (n + DD_TILE - 1) / DD_TILE;
     const uint64_t total = (uint64_t)ntiles * nblk;
-    hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(DD_THREADS), 0, s, data, pitch, nchans,
-                       phase, bases, plan, pw, ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16,
-                       m0, inv_s0, partial, RB);
+    hipLaunchKernelGGL(k_dedisp_u8, dim3((unsigned)total), dim3(DD_THREADS), 0, s, data, pitch, phase, plan, pw,
+                       ntrials, (uint32_t)nblk, ntiles, n, out, out_pitch, fp16, m0, inv_s0, partial);
     return true;
 }
 
@@ -821,8 +798,8 @@ __global__ __launch_bounds__(256) void k_delays_multi(const DedispJob* __restric
 __global__ __launch_bounds__(256) void k_dd_plan_multi(const DedispJob* __restrict__ jobs,
                                                        const uint32_t* __restrict__ blk0, uint32_t njobs) {
     const DedispJob& j = jobs[job_of(blk0, njobs, blockIdx.x)];
-    dd_plan_block(j.delays, j.nchans, j.ntrials, j.kmax, j.windows, (uint32_t)DD_ROW_BYTES, j.bases, j.plan,
-                  j.plan_words, blockIdx.x - j.blk0);
+    dd_plan_block(j.delays, j.nchans, j.ntrials, j.kmax, (uint32_t)DD_ROW_BYTES, j.bases, j.plan, j.plan_words,
+                  blockIdx.x - j.blk0);
 }
 
 // Persistent dedispersion over the work items of many observations: item i of
@@ -869,7 +846,7 @@ __global__ __launch_bounds__(DD_THREADS, 4) void k_dedisp_queue(const DedispJob*
             if (tile >= j.ntiles) break;
             if (k) __syncthreads();
             dd_item(j.data, j.pitch, j.phase_slots, j.plan, j.plan_words, j.ntrials,
-                    j.ntiles, j.n, j.out, j.out_pitch, fp16, j.m0, j.inv_s0, j.partial, j.RB, tile, blk);
+                    j.ntiles, j.n, j.out, j.out_pitch, fp16, j.m0, j.inv_s0, j.partial, tile, blk);
         }
     }
 }
@@ -893,26 +870,10 @@ void launch_dedisp_batch(hipStream_t s, const DedispJob* jobs, const uint32_t* e
                        total_items, span, fp16, counters);
 }
 
-bool dedisp_job_setup(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uint32_t* spread,
-                      DedispJob* j) {
-    if (nchans == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
-    if (pitch % 16 || pitch < 16 || pitch >= (1ull << 31) || nchans > 0xffff) return false;
-    uint32_t smax = 0;
-    for (uint32_t c = 0; c < nchans; c++) smax = std::max(smax, spread[c]);
-    const uint64_t wmax = (uint64_t)smax + 6 + DD_TILE / 8 + 3;
-    const uint32_t RB = (uint32_t)(((uint64_t)smax + 600 + 1023) / 1024 * 1024);
-    if (RB > DD_RAW_BYTES) return false;
-    const uint32_t windows = std::min<uint32_t>(DDP_MAX_WINDOWS, DD_RAW_BYTES / RB);
-    if (windows < 2 || 16 * wmax > (uint64_t)DD_ROW_BYTES) return false;
-    uint32_t kmax = DDP_KMAX;
-    while (kmax > 1 && (uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax--;
-    if ((uint64_t)kmax * 16 * wmax > (uint64_t)DD_ROW_BYTES) kmax = 0;
-    if (ddp_plan_shm(nchans) > 160 * 1024) return false;
-    uint32_t phase = 65535u / (2 * (maxval ? maxval : 1));
-    phase = phase > 4096 ? 4096 : phase;
-    if (phase < 2) return false;
-    j->RB = RB;
-    j->windows = windows;
+bool dedisp_job_setup(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uint32_t* spread, uint32_t n,
+                      uint64_t max_delay, DedispJob* j) {
+    uint32_t kmax = 0, phase = 0;
+    if (!dd_shape(nchans, maxval, pitch, spread, n, max_delay, &kmax, &phase)) return false;
     j->kmax = kmax;
     j->phase_slots = phase;
     j->plan_words = ddp_plan_words(nchans);
