@@ -110,6 +110,33 @@ def test_dedisperse_full_length_and_unsorted_dms(ctx):
         assert (got[i] == ref).all()
 
 
+@pytest.mark.parametrize("nchans,step", [(97, 0.25), (97, 3.0), (96, 0.5)])
+def test_dedisperse_reads_to_the_end_of_the_data(ctx, nchans, step):
+    """The row build reads each channel a few hundred samples past the last
+    delayed sample it sums: with descending frequencies the last channel has the
+    largest delays, and at the full available output length (the largest DM's
+    delay plus n reaching the last sample, nsamples a multiple of 256 so the row
+    pitch has its minimum slack) its reads run to the end of the allocation.
+    Odd channel counts leave the last channel as a single-channel row; fine and
+    coarse DM steps give pair rows and split pairs -- bit-exact."""
+    rng = np.random.default_rng(nchans * 7 + int(step * 4))
+    ns = 3 << 13
+    payload = rng.integers(0, 256, (ns, nchans), dtype=np.uint8).tobytes()
+    foff = -200.0 / nchans
+    obs = dedisperse.Observation.from_packed(payload, 8, nchans, 64e-6, 1500.0, foff)
+    data, ns2 = oracle.decode(payload, 8, nchans)
+    freqs = 1500.0 + foff * np.arange(nchans)
+    dms = np.arange(80) * step
+    print("dedisperse %d channels, %d trials of step %g to the last sample" % (nchans, len(dms), step), flush=True)
+    got = dedisperse.apply_dms(obs, dms)  # None: the full available length
+    dmax = int(oracle.delays(freqs, 64e-6, float(dms[-1])).max())
+    assert got.shape[1] == ns - dmax
+    for i, dm in enumerate(dms):
+        ref = oracle.apply(data, ns2, oracle.delays(freqs, 64e-6, float(dm)), got.shape[1])
+        assert (got[i].view(np.uint32) == ref.view(np.uint32)).all(), "DM %g" % dm
+    obs.close()
+
+
 def test_apply_delay_table_matches_oracle(ctx):
     """psr_apply, 1:1 with dedisperse::apply(data, nsamples, delays, length): the
     full band with a DM's delays, subband channel slices with the full band's
