@@ -753,14 +753,38 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
             }
         }
     } else {
+        // all 16 powers first (the swizzles need every lane), then one predicated
+        // run of stores from one base address; the floor's partial sums the
+        // thread's 16 values in f32 first (16 positive terms: relative error
+        // below 1e-6 of a floor that is 1e-6 of the mean power)
         const bool keep = row == 0 || r1 != r2;
+        float pw[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             const float2 s = v[15 - q];
             const float2 zp = make_float2(swz16(s.x), swz16(s.y));
-            const float p = r2c_power(v[q], zp, cmul(base, w32(q)));
-            const int k = k1 + 16 * q1 + 256 * q;
-            if (keep) acc += store_power<F16>(spec, srow + (size_t)rowk * M1 + k, p, inv_n);
+            pw[q] = r2c_power(v[q], zp, q == 0 ? base : cmul(base, w32(q)));
+        }
+        if (keep) {
+            const size_t o = srow + (size_t)rowk * M1 + k1 + 16 * q1;
+            float part = 0.f;
+            if (F16) {
+                __half* dst = (__half*)spec + o;
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    const float x = pw[q] * inv_n;
+                    dst[256 * q] = __float2half_rn(__builtin_amdgcn_sqrtf(x));
+                    part += x;
+                }
+            } else {
+                float* dst = (float*)spec + o;
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    dst[256 * q] = pw[q];
+                    part += pw[q];
+                }
+            }
+            acc = (double)part;
         }
     }
     const double sum = block_sum(acc, red);
