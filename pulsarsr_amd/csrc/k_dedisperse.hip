@@ -470,8 +470,8 @@ static_assert(2 * DD_THREADS >= DDP_STAGE_PITCH, "a record is two dwords per thr
 // k < 8, g = the quad's index in its row: aligned) and eight of channel c + 1
 // (samples off_c1 + 4 g + 64 k: unaligned dword loads), read from the channel
 // rows in global memory (L2-resident: the workgroups of one XCD work on the
-// same time tile), and stores them in a lane-rotated order, which spreads
-// every 16 lanes' stores over all LDS banks.  Single-channel rows read their
+// same time tile), and stores them in a lane-rotated order that keeps the
+// 16-byte stores of every 8 or 16 lanes on distinct LDS banks.  Single-channel rows read their
 // own channel twice and zero the second copy's bytes with an all-zero byte
 // select.
 __device__ __forceinline__ void dd_build(const uint32_t* sr, uint32_t* rows, const uint8_t* __restrict__ tile_base,
@@ -491,7 +491,10 @@ __device__ __forceinline__ void dd_build(const uint32_t* sr, uint32_t* rows, con
             d[k] = *(const uint32_t*)(pc + 64 * k);
             __builtin_memcpy(&e[k], p1 + 64 * k, 4);
         }
-        const uint32_t rot = (q >> 2) & 3u;
+        // store order of the four words: lanes q with equal q % 4 within 16
+        // lanes, and the even (odd) lanes within 8, start at different words,
+        // so the 16-byte stores of 8 or 16 lanes at a time hit distinct banks
+        const uint32_t rot = ((q >> 1) + (q >> 3)) & 3u;
         uint32_t* row = rows + 16 * q;
 #pragma unroll
         for (uint32_t bb = 0; bb < 4; bb++) {
