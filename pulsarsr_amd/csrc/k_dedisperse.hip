@@ -687,7 +687,12 @@ __device__ __forceinline__ void dd_item(const uint8_t* __restrict__ data, uint64
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
         __syncthreads();
         if (st + 1 < nstages) dd_rec_store(srec[(st + 1) % DD_RECS], rnext, threadIdx.x);
+        // the build's loads wait on memory while the other workgroup of the CU
+        // sums from the LDS: issuing them first (wave priority 1) measured ~1 ms
+        // faster per headline launch
+        __builtin_amdgcn_s_setprio(1);
         dd_build(sr, lds, data + t0, pitch, threadIdx.x, DD_THREADS);
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         rnext = dd_rec_load(pb, min(st + 2, nstages - 1), threadIdx.x);
         sums.stage(sr, lds_lane, wave, phase_slots);
