@@ -29,7 +29,12 @@ HBM_PEAK_GBS = 8000.0
 # holds two outputs' sums of a channel pair (one byte per add), and v_add3_u32
 # adds two such words into u16-pair accumulators (eight adds per lane-op).
 LDS_PEAK_GBS = 256 * 256 * 2.4e9 / 1e9
+LDS_PROBE_GBS = 256 * 190 * 2.4e9 / 1e9  # the ds_read_b128 rate a probe reached
 VALU_PEAK_TADDS = 256 * 64 * 8 * 2.4e9 / 1e12
+# FP32 VALU: 157.3 TFLOP/s with packed FP32 instructions, which this build leaves
+# disabled (one FMA per lane per clock: 78.6 TFLOP/s)
+FP32_PEAK_TFLOPS = 157.3
+FP32_UNPACKED_TFLOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 LDS_BYTES_PER_ADD = 1.0
 
 
@@ -346,6 +351,8 @@ def main():
     spec_alg = 2 + 2 * r + 2 * r + 2 * r
     spec_parts = [traffic_gb(k, w, launches) for k in ("k_fft_pass1r", "k_fft_pass2r", "k_whiten_fused16", "k_harmsum")]
     spec_traffic = sum(spec_parts) if all(x is not None for x in spec_parts) else None
+    nfft = float(stages["fft_len"])
+    fft_flop = len(w.dms) * 2.5 * nfft * np.log2(nfft)
     res = {
         "metric": "DM-trial·samples/s (dedisp+FFT+harmsum), FP16-store vs FP32 at same candidates",
         "value": units / (ms16 * 1e-3),
@@ -366,6 +373,7 @@ def main():
         "ingest_s": ingest_s,
         "roofline": {"bound": "lds", "kernel": "k_dedisp_u8", "achieved": ach_lds, "peak": LDS_PEAK_GBS,
                      "unit": "GB/s", "frac": ach_lds / LDS_PEAK_GBS,
+                     "probe_peak": LDS_PROBE_GBS, "frac_of_probe_peak": ach_lds / LDS_PROBE_GBS,
                      "traffic": dd_traffic * 1e9 if dd_traffic else None,
                      "valu": {"achieved": ach_tadds, "peak": VALU_PEAK_TADDS, "unit": "Tadd/s",
                               "frac": ach_tadds / VALU_PEAK_TADDS},
@@ -378,6 +386,14 @@ def main():
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": (units * 2.0 + w.nchans * w.nsamples) / (dd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                              "note": "algorithmic bytes: FP16 series write + the filterbank read once"}},
+        # the FFT is VALU-bound: 2.5 N log2 N flop per real transform of N points
+        "roofline_fft": {"bound": "valu", "kernels": "k_fft_pass1r+k_fft_pass2r",
+                         "achieved": fft_flop / (stages["fft_ms"] * 1e-3) / 1e12, "unit": "TFLOP/s",
+                         "peak": FP32_PEAK_TFLOPS,
+                         "frac": fft_flop / (stages["fft_ms"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                         "frac_unpacked": fft_flop / (stages["fft_ms"] * 1e-3) / 1e12 / FP32_UNPACKED_TFLOPS,
+                         "note": "flop = trials x 2.5 N log2 N (N = FFT length); peak 157.3 TFLOP/s counts packed "
+                                 "FP32, which this build disables; frac_unpacked is against 78.6 TFLOP/s"},
         "roofline_spectral": {"bound": "hbm", "kernels": "fft+whiten+harmsum",
                               "achieved": float(units) * spec_alg / (spec_ms * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
