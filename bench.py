@@ -84,10 +84,10 @@ def top_families(cands, n, tsamp, k=20):
 # that launch shape only: other workloads or trial counts report traffic null.
 MEASURED_LAUNCH = {"workload": "headline-1024ch-8bit", "trials": 4000, "launches": 1}
 MEASURED_TRAFFIC_GB_PER_LAUNCH = {
-    "k_dedisp_u8": 86.47 + 17.64,
+    "k_dedisp_u8": 85.74 + 17.64,
     "k_fft_pass1r": 16.78 + 67.11,
-    "k_fft_pass2r": 67.31 + 16.81,
-    "k_whiten_fused16": 31.90 + 17.36,
+    "k_fft_pass2r": 67.29 + 16.81,
+    "k_whiten_fused16": 31.87 + 17.36,
     "k_harmsum": 15.22 + 0.02,
 }
 
@@ -379,8 +379,8 @@ def main():
                               "frac": ach_tadds / VALU_PEAK_TADDS},
                      "note": "1 LDS byte per add (channel-pair rows), adds = trials x series_len x nchans; "
                              "HIP-event time of the dedispersion stage; traffic = HBM bytes of this launch "
-                             "(FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_headline-1024ch-8bit.md): 10.4 of "
-                             "12.5 B/unit are reads, the stage records (0.18 MB per work item from the plan "
+                             "(FETCH_SIZE + WRITE_SIZE, profiles/hbm_traffic_headline-1024ch-8bit.md): 10.3 of "
+                             "12.4 B/unit are reads, the stage records (0.18 MB per work item from the plan "
                              "table) and the channel bytes the row build reads, fetched again after L2 eviction",
                      "hbm": {"achieved": (units * 2.0 + w.nchans * w.nsamples) / (dd_ms * 1e-3) / 1e9,
                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
