@@ -732,7 +732,7 @@ static bool dd_shape(uint32_t nchans, uint32_t maxval, uint64_t pitch, const uin
     // aligned dword loads of channel rows; 16-bit channel numbers
     if (nchans == 0 || n == 0 || (uint64_t)nchans * maxval >= (1u << 20)) return false;
     if (pitch % 16 || pitch < 16 || nchans > 0xffff) return false;
-    if ((uint64_t)n - 1 + max_delay + DD_READ_REACH > pitch + DD_ALLOC_SLACK) return false;
+    if (max_delay > pitch || (uint64_t)n - 1 + max_delay + DD_READ_REACH > pitch + DD_ALLOC_SLACK) return false;
     uint32_t smax = 0;
     for (uint32_t c = 0; c < nchans; c++) smax = std::max(smax, spread[c]);
     const uint64_t wmax = (uint64_t)smax + 6 + DD_TILE / 8 + 3;
