@@ -595,6 +595,19 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass1(const void* __restrict__ s
 }
 
 // --------------------------------------------------------------- pass 2
+// Row pairs (r1, m2 - r1) per workgroup for row length M1 < 4096: enough that a
+// workgroup holds 8192 points (every thread has whole butterflies in each
+// Stockham pass), at most 256.
+constexpr int pass2_pairs(int m1) { return m1 >= 4096 ? 1 : (4096 / m1 > 256 ? 256 : 4096 / m1); }
+constexpr int pass2_elems(int m1) { return 2 * pass2_pairs(m1) * m1; }
+constexpr int pass2_buf(int m1) { return pass2_elems(m1) + (pass2_elems(m1) >> 4) + 1; }
+
+// P = pass2_pairs(M1) row pairs, pair p = rows r1 = P blockIdx.x + p and
+// r2 = m2 - r1 (r1 <= m2 / 2), as 2P interleaved vectors through the LDS
+// Stockham passes; then every bin's r2c power (the partner of bin k1 of row r1
+// is bin M1 - 1 - k1 of row r2, of row 0 bin M1 - k1 of itself) and each
+// pair's sum of the stored values for the whitening floor, reduced in a fixed
+// order (pairwise over the pair's bins in LDS).
 template <int M1, bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__ work, uint32_t n_all,
                                                    const uint32_t* __restrict__ trial_n,
@@ -603,73 +616,118 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__
                                                    double* __restrict__ partial,
                                                    const float2* __restrict__ twrow,
                                                    const float* __restrict__ cos_tables) {
-    constexpr int NBUF = 2 * M1 + ((2 * M1) >> 4) + 1;
+    constexpr int P = pass2_pairs(M1), V = 2 * P, NE = pass2_elems(M1), NBUF = pass2_buf(M1);
     constexpr int NT = tw_total(M1) + 1;
-    constexpr int PER = cdiv(M1, FFT_THREADS);
-    constexpr int PER2 = cdiv(2 * M1, FFT_THREADS);
+    constexpr int PE = NE / FFT_THREADS, NW = FFT_THREADS / 64;
+    static_assert(NE % FFT_THREADS == 0, "whole values per thread");
+    static_assert(P * NW <= NBUF && PE * NW <= NBUF, "the wave sums reuse the transform buffer");
     extern __shared__ float2 smem[];
-    float2* buf = smem;                    // 2 rows interleaved, padded
+    float2* buf = smem;  // the 2P rows interleaved, padded
     float2* T1 = smem + NBUF;
     float2* T4 = T1 + NT;
     const float* C2 = cos_tables + costab_offset(2 * M1);  // W_N^(m2 k1) = W_(2 M1)^k1
-    __shared__ double red[FFT_THREADS / 64];
-    const uint32_t trial = blockIdx.y;
+    const uint32_t trial = blockIdx.y, npairs = m2 / 2 + 1, p0 = blockIdx.x * P;
     const uint32_t n = trial_n ? trial_n[trial] : n_all;
-    const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
     const float2* w = work + (size_t)trial * M;
-    float2 va[PER], vb[PER];
+    // vector v holds row r1 (v even) or r2 (v odd) of pair v / 2
+    auto row_of = [&](int v) {
+        const uint32_t r1 = p0 + (uint32_t)(v >> 1);
+        return (v & 1) ? (m2 - r1) & (m2 - 1) : r1;
+    };
+    float2 va[PE];
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const int k1 = threadIdx.x + i * FFT_THREADS;
-        if (M1 % FFT_THREADS == 0 || k1 < M1) {
-            va[i] = w[(size_t)r1 * M1 + k1];
-            vb[i] = w[(size_t)r2 * M1 + k1];
-        }
+    for (int i = 0; i < PE; i++) {
+        const int j = threadIdx.x + i * FFT_THREADS, v = j / M1, k1 = j % M1;
+        va[i] = p0 + (uint32_t)(v >> 1) < npairs ? w[(size_t)row_of(v) * M1 + k1] : make_float2(0.f, 0.f);
     }
     if constexpr (M1 >= 4) build_pass_tw<M1>(T1, T4, cos_tables);
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const int k1 = threadIdx.x + i * FFT_THREADS;
-        if (M1 % FFT_THREADS == 0 || k1 < M1) {
-            buf[lidx<2, true>(0, k1)] = va[i];
-            buf[lidx<2, true>(1, k1)] = vb[i];
-        }
+    for (int i = 0; i < PE; i++) {
+        const int j = threadIdx.x + i * FFT_THREADS;
+        buf[lidx<V, true>(j / M1, j % M1)] = va[i];
     }
     __syncthreads();
-    fft_passes<M1, 2, true, 0, 1>(buf, T1, T4);
+    fft_passes<M1, V, true, 0, 1>(buf, T1, T4);
     const float inv_n = 1.0f / (float)n;
     const size_t srow = (size_t)trial * spec_pitch;
-    double acc = 0.0;
-    const int nsel = r1 == r2 ? 1 : 2;
-    const float2 wr1 = twrow[r1], wr2 = twrow[r2];
-    float2 ck[PER2];
-#pragma unroll
-    for (int i = 0; i < PER2; i++) {
-        const int idx = threadIdx.x + i * FFT_THREADS;
-        ck[i] = idx < nsel * M1 ? twiddle<2 * M1>(C2, idx % M1) : make_float2(0.f, 0.f);
-    }
-#pragma unroll
-    for (int i = 0; i < PER2; i++) {
-        const int idx = threadIdx.x + i * FFT_THREADS;
-        if (idx >= nsel * M1) break;
-        const int sel = idx / M1, k1 = idx % M1;
+    // the stored values (f32, exactly what the floor sums) and the Nyquist one
+    float val[PE];
+    float nyq = 0.f;
+#pragma unroll 4
+    for (int i = 0; i < PE; i++) {
+        const int j = threadIdx.x + i * FFT_THREADS;
+        val[i] = 0.f;
+        const int v = j / M1, k1 = j % M1, pr = v >> 1, sel = v & 1;
+        const uint32_t r1 = p0 + (uint32_t)pr, r2 = (m2 - r1) & (m2 - 1);
+        const int nsel = r1 == r2 ? 1 : 2;
+        if (r1 >= npairs || sel >= nsel) continue;
         const uint32_t rowk = sel ? r2 : r1;
-        const float2 zk = buf[lidx<2, true>(sel, k1)];
+        const float2 zk = buf[lidx<V, true>(v, k1)];
         float2 zp;
         if (rowk == 0)
-            zp = buf[lidx<2, true>(0, (M1 - k1) & (M1 - 1))];
+            zp = buf[lidx<V, true>(2 * pr, (M1 - k1) & (M1 - 1))];
         else
-            zp = buf[lidx<2, true>(nsel == 1 ? 0 : 1 - sel, M1 - 1 - k1)];
-        const float2 wk = cmul(sel ? wr2 : wr1, ck[i]);
-        const float p = r2c_power(zk, zp, wk);
-        acc += store_power<F16>(spec, srow + (size_t)rowk * M1 + k1, p, inv_n);
+            zp = buf[lidx<V, true>(2 * pr + (nsel == 1 ? 0 : 1 - sel), M1 - 1 - k1)];
+        const float2 wk = cmul(twrow[rowk], twiddle<2 * M1>(C2, k1));
+        const float pw = r2c_power(zk, zp, wk);
+        val[i] = (float)store_power<F16>(spec, srow + (size_t)rowk * M1 + k1, pw, inv_n);
         if (rowk == 0 && k1 == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
             const float xm = zk.x - zk.y;
-            acc += store_power<F16>(spec, srow + M, xm * xm, inv_n);
+            nyq = (float)store_power<F16>(spec, srow + M, xm * xm, inv_n);
         }
     }
-    const double s = block_sum(acc, red);
-    if (threadIdx.x == 0) partial[(size_t)trial * (m2 / 2 + 1) + r1] = s;
+    // each pair's 2 M1 stored values (zeros where a bin is not stored) summed in
+    // a fixed order: within a thread, across the lanes, then across the waves
+    constexpr int G2 = 2 * M1;  // values of one pair
+    const bool nq = p0 == 0 && threadIdx.x == 0;  // row 0's bin 0: takes the Nyquist bin
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double* red = (double*)smem;  // [value group][wave]
+    double* out = partial + (size_t)trial * npairs + p0;
+    if constexpr (G2 > 64) __syncthreads();  // the transform buffer becomes the wave sums
+    if constexpr (G2 >= FFT_THREADS) {
+        // a thread's values of pair q are val[q G .. q G + G - 1]
+        constexpr int G = G2 / FFT_THREADS;
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+            double t = 0.0;
+#pragma unroll
+            for (int g = 0; g < G; g++) t += (double)val[q * G + g];
+            if (q == 0 && nq) t += (double)nyq;
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+            if (lane == 0) red[q * NW + wv] = t;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < P; q += FFT_THREADS) {
+            double t = 0.0;
+            for (int k = 0; k < NW; k++) t += red[q * NW + k];
+            if (p0 + (uint32_t)q < npairs) out[q] = t;
+        }
+    } else {
+        // val[i] of thread x belongs to pair i PS + x / G2: G2 consecutive lanes
+        constexpr int PS = FFT_THREADS / G2, SEG = G2 < 64 ? G2 : 64;
+#pragma unroll
+        for (int i = 0; i < PE; i++) {
+            double t = (double)val[i];
+            if (i == 0 && nq) t += (double)nyq;
+            for (int o = SEG / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
+            if constexpr (G2 <= 64) {
+                const int q = i * PS + (int)threadIdx.x / G2;
+                if ((lane & (G2 - 1)) == 0 && p0 + (uint32_t)q < npairs) out[q] = t;
+            } else if (lane == 0) {
+                red[i * NW + wv] = t;
+            }
+        }
+        if constexpr (G2 > 64) {
+            constexpr int W = G2 / 64;  // waves per pair
+            __syncthreads();
+            for (int q = threadIdx.x; q < P; q += FFT_THREADS) {
+                const int i = q / PS, w0 = (q % PS) * W;
+                double t = 0.0;
+                for (int k = 0; k < W; k++) t += red[i * NW + w0 + k];
+                if (p0 + (uint32_t)q < npairs) out[q] = t;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------- pass 2, register-first
@@ -823,10 +881,10 @@ static void launch_pass2(hipStream_t s, const SpecGeom& g, const uint32_t* trial
                            s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
         return;
     }
-    const size_t shm =
-        (size_t)(2 * M1 + ((2 * M1) >> 4) + 1 + 2 * (tw_total(M1) + 1)) * sizeof(float2);
+    const size_t shm = (size_t)(pass2_buf(M1) + 2 * (tw_total(M1) + 1)) * sizeof(float2);
     allow_lds(k_fft_pass2<M1, F16>, shm);
-    hipLaunchKernelGGL((k_fft_pass2<M1, F16>), dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), shm,
+    const uint32_t npairs = g.m2 / 2 + 1, P = (uint32_t)pass2_pairs(M1);
+    hipLaunchKernelGGL((k_fft_pass2<M1, F16>), dim3((npairs + P - 1) / P, ntrials), dim3(FFT_THREADS), shm,
                        s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
 }
 
