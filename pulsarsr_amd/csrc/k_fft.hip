@@ -854,6 +854,174 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
     if (threadIdx.x == 0) partial[(size_t)trial * (m2 / 2 + 1) + r1] = sum;
 }
 
+// ------------------------------------- pass 2, register-first, m1 = 256 R3
+// The three stages of k_fft_pass2r for m1 = 256 R3 (R3 = 1, 2, 4, 8), on
+// NR = 32 / R3 row vectors (P = NR / 2 pairs r1 = P blockIdx.x + p, r2 =
+// m2 - r1), 16 points per thread.  Row element n = tt + S m (S = 16 R3):
+// stage A is the radix 16 over m (output k1) with W_m1^(tt k1); with
+// tt = a + 16 b, stage B1 the radix R3 over b (output q1) with W_S^(a q1);
+// stage B2 the radix 16 over a (output q2), bin k = k1 + 16 q1 + S q2.  In
+// stage B2 lane l holds vector 2 p + ((l >> 4) & 1) and, for r2, the mirrored
+// (15 - k1, R3 - 1 - q1) of lane l ^ 16's r1 bins, so the r2c partner
+// m1 - 1 - k of every bin is a swizzle away; pair 0 (row 0 with itself,
+// partner m1 - k) takes one more LDS exchange.  Every 16 lanes touch 16
+// distinct LDS bank pairs in each exchange.
+template <int R3, bool F16>
+__global__ __launch_bounds__(512, 4) void k_fft_pass2x(const float2* __restrict__ work, uint32_t n_all,
+                                                    const uint32_t* __restrict__ trial_n,
+                                                    uint32_t M, uint32_t m2,
+                                                    void* __restrict__ spec, uint64_t spec_pitch,
+                                                    double* __restrict__ partial,
+                                                    const float2* __restrict__ twrow,
+                                                    const float* __restrict__ cos_tables) {
+    constexpr int M1 = 256 * R3, S = 16 * R3, NR = 32 / R3, P = NR / 2;
+    static_assert(R3 == 1 || R3 == 2 || R3 == 4 || R3 == 8, "m1 = 256, 512, 1024 or 2048");
+    __shared__ float2 buf[NR * M1];
+    __shared__ float2 TA[S];  // W_S^(a q1) at q1 * 16 + a
+    __shared__ double red[FFT_THREADS / 64];
+    const uint32_t trial = blockIdx.y, npairs = m2 / 2 + 1, p0 = blockIdx.x * P;
+    const uint32_t n = trial_n ? trial_n[trial] : n_all;
+    const float2* w = work + (size_t)trial * M;
+    const int t = threadIdx.x;
+    float2 v[16];
+    {  // stage A: radix 16 over m, twiddle W_m1^(tt k1) = W_4096^(tt (16 / R3) k1)
+        const int vec = t / S, tt = t % S;
+        const uint32_t r1 = p0 + (uint32_t)(vec >> 1);
+        if (r1 < npairs) {
+            const uint32_t row = (vec & 1) ? (m2 - r1) & (m2 - 1) : r1;
+            const float2* src = w + (size_t)row * M1 + tt;
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = src[S * m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
+        }
+        const float2* T = (const float2*)(cos_tables + TW4096_OFFSET) + tt * (16 / R3);
+        float2 tw[15];
+#pragma unroll
+        for (int k = 0; k < 15; k++) tw[k] = T[256 * k];
+        if constexpr (R3 > 1)
+            if (t < S) TA[t] = twiddle<S>(cos_tables + costab_offset(S), (t >> 4) * (t & 15));
+        dft<16>(v);
+#pragma unroll
+        for (int k = 1; k < 16; k++) v[k] = cmul(v[k], tw[k - 1]);
+        if constexpr (R3 == 1) {  // straight into the stage-B2 layout (a = tt, q1 = 0)
+#pragma unroll
+            for (int k = 0; k < 16; k++) buf[vec * M1 + k * 16 + (tt ^ k)] = v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; k++) buf[vec * M1 + k * S + tt] = v[k];
+        }
+    }
+    __syncthreads();
+    if constexpr (R3 > 1) {  // stage B1: 16 / R3 radix-R3 butterflies (vector, k1, a) per thread
+        constexpr int NB = 16 / R3;
+        const int a = t & 15, k1 = (t >> 4) & 15;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int vec = (t >> 8) + 2 * i;
+#pragma unroll
+            for (int b = 0; b < R3; b++) v[i * R3 + b] = buf[vec * M1 + k1 * S + 16 * b + a];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            dft<R3>(v + i * R3);
+#pragma unroll
+            for (int q = 1; q < R3; q++) v[i * R3 + q] = cmul(v[i * R3 + q], TA[q * 16 + a]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int vec = (t >> 8) + 2 * i;
+#pragma unroll
+            for (int q = 0; q < R3; q++) buf[vec * M1 + q * 256 + k1 * 16 + (a ^ k1)] = v[i * R3 + q];
+        }
+        __syncthreads();
+    }
+    // stage B2: radix 16 over a
+    const int l = t & 63, sel = (l >> 4) & 1, i16 = l & 15, c = 2 * (t >> 6) + (l >> 5);
+    const int pr = c / R3, qq = c % R3, vec = 2 * pr + sel;
+    const int k1 = sel ? 15 - i16 : i16, q1 = sel ? R3 - 1 - qq : qq;
+#pragma unroll
+    for (int a = 0; a < 16; a++) v[a] = buf[vec * M1 + q1 * 256 + k1 * 16 + (a ^ k1)];
+    dft<16>(v);  // v[q2] = X_row[k1 + 16 q1 + S q2]
+    const float inv_n = 1.0f / (float)n;
+    const size_t srow = (size_t)trial * spec_pitch;
+    const uint32_t r1 = p0 + (uint32_t)pr, r2 = (m2 - r1) & (m2 - 1), rowk = sel ? r2 : r1;
+    const int kb = k1 + 16 * q1;
+    const float2 base = cmul(twrow[rowk], twiddle<2 * M1>(cos_tables + costab_offset(2 * M1), kb));
+    double acc = 0.0;
+    // all 16 powers first (the swizzles need every lane); pair 0's are redone below
+    float pw[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const float2 s = v[15 - q];
+        const float2 zp = make_float2(swz16(s.x), swz16(s.y));
+        pw[q] = r2c_power(v[q], zp, q == 0 ? base : cmul(base, w32(q)));
+    }
+    if (p0 == 0) {  // row 0 twice: the partner of k is m1 - k in the same row
+        __syncthreads();
+        if (pr == 0 && sel == 0)
+#pragma unroll
+            for (int q = 0; q < 16; q++) buf[kb + S * q] = v[q];
+        __syncthreads();
+        if (pr == 0 && sel == 0) {
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int k = kb + S * q;
+                const float2 zk = v[q], zp = buf[(M1 - k) & (M1 - 1)];
+                const float p = r2c_power(zk, zp, cmul(base, w32(q)));
+                acc += store_power<F16>(spec, srow + k, p, inv_n);
+                if (k == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
+                    const float xm = zk.x - zk.y;
+                    acc += store_power<F16>(spec, srow + M, xm * xm, inv_n);
+                }
+            }
+        }
+    }
+    // one predicated run of stores per thread; the floor's partial sums the
+    // thread's 16 values in f32 first (16 positive terms)
+    const bool keep = !(p0 == 0 && pr == 0) && r1 < npairs && (sel == 0 || r1 != r2);
+    if (keep) {
+        const size_t o = srow + (size_t)rowk * M1 + kb;
+        float part = 0.f;
+        if (F16) {
+            __half* dst = (__half*)spec + o;
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float x = pw[q] * inv_n;
+                dst[S * q] = __float2half_rn(__builtin_amdgcn_sqrtf(x));
+                part += x;
+            }
+        } else {
+            float* dst = (float*)spec + o;
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                dst[S * q] = pw[q];
+                part += pw[q];
+            }
+        }
+        acc = (double)part;
+    }
+    // pair p's values are those of threads 32 R3 p .. 32 R3 (p + 1) - 1
+    constexpr int SEG = 32 * R3;
+    double* out = partial + (size_t)trial * npairs + p0;
+    if constexpr (SEG <= 64) {
+        for (int o = SEG / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if ((t & (SEG - 1)) == 0 && r1 < npairs) out[pr] = acc;
+    } else {
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (l == 0) red[t >> 6] = acc;
+        __syncthreads();
+        if (t < P && p0 + (uint32_t)t < npairs) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < SEG / 64; k++) sum += red[t * (SEG / 64) + k];
+            out[t] = sum;
+        }
+    }
+}
+
 // ------------------------------------------------------------- launchers
 template <typename K>
 static void allow_lds(K kernel, size_t bytes) {
@@ -879,13 +1047,18 @@ static void launch_pass2(hipStream_t s, const SpecGeom& g, const uint32_t* trial
     if constexpr (M1 == 4096) {
         hipLaunchKernelGGL((k_fft_pass2r<F16>), dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), 0,
                            s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
-        return;
+    } else if constexpr (M1 >= 256) {
+        constexpr uint32_t P = 8192 / M1 / 2;
+        hipLaunchKernelGGL((k_fft_pass2x<M1 / 256, F16>), dim3((g.m2 / 2 + 1 + P - 1) / P, ntrials),
+                           dim3(FFT_THREADS), 0, s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial,
+                           twrow, cos_tables);
+    } else {
+        const size_t shm = (size_t)(pass2_buf(M1) + 2 * (tw_total(M1) + 1)) * sizeof(float2);
+        allow_lds(k_fft_pass2<M1, F16>, shm);
+        const uint32_t npairs = g.m2 / 2 + 1, P = (uint32_t)pass2_pairs(M1);
+        hipLaunchKernelGGL((k_fft_pass2<M1, F16>), dim3((npairs + P - 1) / P, ntrials), dim3(FFT_THREADS), shm,
+                           s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
     }
-    const size_t shm = (size_t)(pass2_buf(M1) + 2 * (tw_total(M1) + 1)) * sizeof(float2);
-    allow_lds(k_fft_pass2<M1, F16>, shm);
-    const uint32_t npairs = g.m2 / 2 + 1, P = (uint32_t)pass2_pairs(M1);
-    hipLaunchKernelGGL((k_fft_pass2<M1, F16>), dim3((npairs + P - 1) / P, ntrials), dim3(FFT_THREADS), shm,
-                       s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
 }
 
 template <int M2, bool F16>

@@ -226,8 +226,10 @@ def test_f32_channel_order_sums(ctx):
         assert (got[i].view(np.uint32) == ref.view(np.uint32)).all()
 
 
-@pytest.mark.parametrize("n", [1000, 4096, 8000, 32583, 100000, 263500, 1500000, 2097152, 2300000, 4194305,
-                               8388608])
+# M = next_pow2(n) complex points: one pass up to 4096, else rows of m1 = M / 512
+# (m1 = 16, 64, 128, 256, 512, 1024, 2048, 4096) and m1 = 4096 with m2 = 1024, 2048
+@pytest.mark.parametrize("n", [1000, 4096, 8000, 32583, 50000, 100000, 200000, 263500, 600000, 1500000, 2097152,
+                               2300000, 4194305, 8388608])
 def test_spectrum_matches_oracle(ctx, n):
     rng = np.random.default_rng(n)
     t = np.arange(n)
