@@ -463,6 +463,29 @@ __device__ __forceinline__ float2 w32(int q) {  // W_32^q, q < 16
 __device__ __forceinline__ float swz16(float x) {  // value of lane l ^ 16 (within 32-lane halves)
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x401f));
 }
+// Powers of the 16 bins v[q] of this lane, whose r2c partners are bins 15 - q
+// of lane l ^ 16 (W_N^k of bin q = base * W_32^q).  With 2E = Z + conj Zp and
+// 2O = (Z - conj Zp) / i, X_k = E + W O and the partner's X = conj(E - W O), so
+// each lane forms the pairs (its q, partner's 15 - q) for q < 8 and returns
+// the partner's 8 powers through the swizzle: half the butterflies and
+// twiddles of forming all 16 from both sides.
+__device__ __forceinline__ void r2c_powers_paired(const float2 (&v)[16], float2 base, float (&pw)[16]) {
+    float pp[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const float2 zk = v[q], s = v[15 - q];
+        const float2 zp = make_float2(swz16(s.x), swz16(s.y));
+        const float2 w = q == 0 ? base : cmul(base, w32(q));
+        const float2 e = make_float2(zk.x + zp.x, zk.y - zp.y);
+        const float2 od = make_float2(zk.y + zp.y, zp.x - zk.x);
+        const float2 t = cmul(w, od);
+        const float2 x1 = cadd(e, t), x2 = csub(e, t);
+        pw[q] = 0.25f * (x1.x * x1.x + x1.y * x1.y);
+        pp[q] = 0.25f * (x2.x * x2.x + x2.y * x2.y);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) pw[15 - q] = swz16(pp[q]);
+}
 template <bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_pass1r(const void* __restrict__ series,
                                                     const float* __restrict__ mean32,
@@ -822,12 +845,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
         // below 1e-6 of a floor that is 1e-6 of the mean power)
         const bool keep = row == 0 || r1 != r2;
         float pw[16];
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const float2 s = v[15 - q];
-            const float2 zp = make_float2(swz16(s.x), swz16(s.y));
-            pw[q] = r2c_power(v[q], zp, q == 0 ? base : cmul(base, w32(q)));
-        }
+        r2c_powers_paired(v, base, pw);
         if (keep) {
             const size_t o = srow + (size_t)rowk * M1 + k1 + 16 * q1;
             float part = 0.f;
@@ -953,12 +971,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2x(const float2* __restrict_
     double acc = 0.0;
     // all 16 powers first (the swizzles need every lane); pair 0's are redone below
     float pw[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const float2 s = v[15 - q];
-        const float2 zp = make_float2(swz16(s.x), swz16(s.y));
-        pw[q] = r2c_power(v[q], zp, q == 0 ? base : cmul(base, w32(q)));
-    }
+    r2c_powers_paired(v, base, pw);
     if (p0 == 0) {  // row 0 twice: the partner of k is m1 - k in the same row
         __syncthreads();
         if (pr == 0 && sel == 0)
