@@ -758,8 +758,9 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2(const float2* __restrict__
 // loads elements tt + 256 m (m < 16) and runs the first radix-16 stage on
 // them; two LDS exchanges regroup for the other two stages.  The last stage
 // places the r2c partner of every output (row r2, column 4095 - k) in lane
-// l ^ 16, which a swizzle reaches.  Row 0 (its own partner at 4096 - k)
-// takes one more LDS exchange.
+// l ^ 16, which a swizzle reaches.  Workgroup 0 takes the two rows that are
+// their own partners, 0 (partner 4096 - k) and m2 / 2 (4095 - k), through one
+// more LDS exchange, so a trial is exactly m2 / 2 workgroups.
 template <bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict__ work, uint32_t n_all,
                                                    const uint32_t* __restrict__ trial_n,
@@ -774,7 +775,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
     __shared__ double red[FFT_THREADS / 64];
     const uint32_t trial = blockIdx.y;
     const uint32_t n = trial_n ? trial_n[trial] : n_all;
-    const uint32_t r1 = blockIdx.x, r2 = (m2 - r1) & (m2 - 1);
+    const uint32_t r1 = blockIdx.x, r2 = r1 == 0 ? m2 / 2 : m2 - r1;
     const float2* w = work + (size_t)trial * M;
     const int t = threadIdx.x;
     float2 v[16];
@@ -819,23 +820,20 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
     const float* C8192 = cos_tables + costab_offset(8192);
     const float2 base = cmul(twrow[rowk], twiddle<8192>(C8192, k1 + 16 * q1));
     double acc = 0.0;
-    if (r1 == 0) {  // row 0 twice: partner of k is 4096 - k in the same row
+    if (r1 == 0) {  // rows 0 (partner 4096 - k) and m2 / 2 (partner 4095 - k)
         __syncthreads();
-        if (row == 0)
 #pragma unroll
-            for (int q = 0; q < 16; q++) buf[k1 + 16 * q1 + 256 * q] = v[q];
+        for (int q = 0; q < 16; q++) buf[row * M1 + k1 + 16 * q1 + 256 * q] = v[q];
         __syncthreads();
-        if (row == 0) {
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int k = k1 + 16 * q1 + 256 * q;
-                const float2 zk = v[q], zp = buf[(M1 - k) & (M1 - 1)];
-                const float p = r2c_power(zk, zp, cmul(base, w32(q)));
-                acc += store_power<F16>(spec, srow + k, p, inv_n);
-                if (k == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
-                    const float xm = zk.x - zk.y;
-                    acc += store_power<F16>(spec, srow + M, xm * xm, inv_n);
-                }
+        for (int q = 0; q < 16; q++) {
+            const int k = k1 + 16 * q1 + 256 * q;
+            const float2 zk = v[q], zp = buf[row ? M1 + (M1 - 1 - k) : (M1 - k) & (M1 - 1)];
+            const float p = r2c_power(zk, zp, cmul(base, w32(q)));
+            acc += store_power<F16>(spec, srow + (size_t)rowk * M1 + k, p, inv_n);
+            if (row == 0 && k == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
+                const float xm = zk.x - zk.y;
+                acc += store_power<F16>(spec, srow + M, xm * xm, inv_n);
             }
         }
     } else {
@@ -843,10 +841,9 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
         // run of stores from one base address; the floor's partial sums the
         // thread's 16 values in f32 first (16 positive terms: relative error
         // below 1e-6 of a floor that is 1e-6 of the mean power)
-        const bool keep = row == 0 || r1 != r2;
         float pw[16];
         r2c_powers_paired(v, base, pw);
-        if (keep) {
+        {
             const size_t o = srow + (size_t)rowk * M1 + k1 + 16 * q1;
             float part = 0.f;
             if (F16) {
@@ -868,22 +865,34 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2r(const float2* __restrict_
             acc = (double)part;
         }
     }
-    const double sum = block_sum(acc, red);
-    if (threadIdx.x == 0) partial[(size_t)trial * (m2 / 2 + 1) + r1] = sum;
+    double* out = partial + (size_t)trial * (m2 / 2 + 1);
+    if (r1 == 0) {  // the two rows' sums apart
+        const double s0 = block_sum(row ? 0.0 : acc, red);
+        __syncthreads();  // red is reused
+        const double s1 = block_sum(row ? acc : 0.0, red);
+        if (threadIdx.x == 0) {
+            out[0] = s0;
+            out[m2 / 2] = s1;
+        }
+    } else {
+        const double sum = block_sum(acc, red);
+        if (threadIdx.x == 0) out[r1] = sum;
+    }
 }
 
 // ------------------------------------- pass 2, register-first, m1 = 256 R3
 // The three stages of k_fft_pass2r for m1 = 256 R3 (R3 = 1, 2, 4, 8), on
 // NR = 32 / R3 row vectors (P = NR / 2 pairs r1 = P blockIdx.x + p, r2 =
-// m2 - r1), 16 points per thread.  Row element n = tt + S m (S = 16 R3):
+// m2 - r1; pair 0 is rows 0 and m2 / 2), 16 points per thread.  Row element n = tt + S m (S = 16 R3):
 // stage A is the radix 16 over m (output k1) with W_m1^(tt k1); with
 // tt = a + 16 b, stage B1 the radix R3 over b (output q1) with W_S^(a q1);
 // stage B2 the radix 16 over a (output q2), bin k = k1 + 16 q1 + S q2.  In
 // stage B2 lane l holds vector 2 p + ((l >> 4) & 1) and, for r2, the mirrored
 // (15 - k1, R3 - 1 - q1) of lane l ^ 16's r1 bins, so the r2c partner
-// m1 - 1 - k of every bin is a swizzle away; pair 0 (row 0 with itself,
-// partner m1 - k) takes one more LDS exchange.  Every 16 lanes touch 16
-// distinct LDS bank pairs in each exchange.
+// m1 - 1 - k of every bin is a swizzle away; pair 0, whose rows are their own
+// partners (row 0: m1 - k, row m2 / 2: m1 - 1 - k), takes one more LDS
+// exchange, so a trial is exactly m2 / (2 P) workgroups.  Every 16 lanes touch
+// 16 distinct LDS bank pairs in each exchange.
 template <int R3, bool F16>
 __global__ __launch_bounds__(512, 4) void k_fft_pass2x(const float2* __restrict__ work, uint32_t n_all,
                                                     const uint32_t* __restrict__ trial_n,
@@ -896,24 +905,20 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2x(const float2* __restrict_
     static_assert(R3 == 1 || R3 == 2 || R3 == 4 || R3 == 8, "m1 = 256, 512, 1024 or 2048");
     __shared__ float2 buf[NR * M1];
     __shared__ float2 TA[S];  // W_S^(a q1) at q1 * 16 + a
-    __shared__ double red[FFT_THREADS / 64];
+    __shared__ double red[2 * (FFT_THREADS / 64)];
     const uint32_t trial = blockIdx.y, npairs = m2 / 2 + 1, p0 = blockIdx.x * P;
     const uint32_t n = trial_n ? trial_n[trial] : n_all;
     const float2* w = work + (size_t)trial * M;
     const int t = threadIdx.x;
+    auto partner_row = [&](uint32_t r1) { return r1 == 0 ? m2 / 2 : m2 - r1; };
     float2 v[16];
     {  // stage A: radix 16 over m, twiddle W_m1^(tt k1) = W_4096^(tt (16 / R3) k1)
         const int vec = t / S, tt = t % S;
         const uint32_t r1 = p0 + (uint32_t)(vec >> 1);
-        if (r1 < npairs) {
-            const uint32_t row = (vec & 1) ? (m2 - r1) & (m2 - 1) : r1;
-            const float2* src = w + (size_t)row * M1 + tt;
+        const uint32_t row = (vec & 1) ? partner_row(r1) : r1;
+        const float2* src = w + (size_t)row * M1 + tt;
 #pragma unroll
-            for (int m = 0; m < 16; m++) v[m] = src[S * m];
-        } else {
-#pragma unroll
-            for (int m = 0; m < 16; m++) v[m] = make_float2(0.f, 0.f);
-        }
+        for (int m = 0; m < 16; m++) v[m] = src[S * m];
         const float2* T = (const float2*)(cos_tables + TW4096_OFFSET) + tt * (16 / R3);
         float2 tw[15];
 #pragma unroll
@@ -965,27 +970,28 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2x(const float2* __restrict_
     dft<16>(v);  // v[q2] = X_row[k1 + 16 q1 + S q2]
     const float inv_n = 1.0f / (float)n;
     const size_t srow = (size_t)trial * spec_pitch;
-    const uint32_t r1 = p0 + (uint32_t)pr, r2 = (m2 - r1) & (m2 - 1), rowk = sel ? r2 : r1;
+    const uint32_t r1 = p0 + (uint32_t)pr, rowk = sel ? partner_row(r1) : r1;
     const int kb = k1 + 16 * q1;
     const float2 base = cmul(twrow[rowk], twiddle<2 * M1>(cos_tables + costab_offset(2 * M1), kb));
     double acc = 0.0;
     // all 16 powers first (the swizzles need every lane); pair 0's are redone below
     float pw[16];
     r2c_powers_paired(v, base, pw);
-    if (p0 == 0) {  // row 0 twice: the partner of k is m1 - k in the same row
+    const bool self = p0 == 0 && pr == 0;  // pair 0: rows 0 and m2 / 2, their own partners
+    if (p0 == 0) {
         __syncthreads();
-        if (pr == 0 && sel == 0)
+        if (self)
 #pragma unroll
-            for (int q = 0; q < 16; q++) buf[kb + S * q] = v[q];
+            for (int q = 0; q < 16; q++) buf[sel * M1 + kb + S * q] = v[q];
         __syncthreads();
-        if (pr == 0 && sel == 0) {
+        if (self) {
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int k = kb + S * q;
-                const float2 zk = v[q], zp = buf[(M1 - k) & (M1 - 1)];
+                const float2 zk = v[q], zp = buf[sel ? M1 + (M1 - 1 - k) : (M1 - k) & (M1 - 1)];
                 const float p = r2c_power(zk, zp, cmul(base, w32(q)));
-                acc += store_power<F16>(spec, srow + k, p, inv_n);
-                if (k == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
+                acc += store_power<F16>(spec, srow + (size_t)rowk * M1 + k, p, inv_n);
+                if (sel == 0 && k == 0) {  // Nyquist bin k = M: X[M] = E[0] - O[0]
                     const float xm = zk.x - zk.y;
                     acc += store_power<F16>(spec, srow + M, xm * xm, inv_n);
                 }
@@ -994,8 +1000,7 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2x(const float2* __restrict_
     }
     // one predicated run of stores per thread; the floor's partial sums the
     // thread's 16 values in f32 first (16 positive terms)
-    const bool keep = !(p0 == 0 && pr == 0) && r1 < npairs && (sel == 0 || r1 != r2);
-    if (keep) {
+    if (!self) {
         const size_t o = srow + (size_t)rowk * M1 + kb;
         float part = 0.f;
         if (F16) {
@@ -1016,21 +1021,47 @@ __global__ __launch_bounds__(512, 4) void k_fft_pass2x(const float2* __restrict_
         }
         acc = (double)part;
     }
-    // pair p's values are those of threads 32 R3 p .. 32 R3 (p + 1) - 1
+    // pair p's values are those of threads 32 R3 p .. 32 R3 (p + 1) - 1; the
+    // sums of its two rows (lanes with sel 0 and 1) are kept apart for pair 0
     constexpr int SEG = 32 * R3;
-    double* out = partial + (size_t)trial * npairs + p0;
+    double* out = partial + (size_t)trial * npairs;
+    double a = sel ? 0.0 : acc, b = sel ? acc : 0.0;
     if constexpr (SEG <= 64) {
-        for (int o = SEG / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if ((t & (SEG - 1)) == 0 && r1 < npairs) out[pr] = acc;
+        for (int o = SEG / 2; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        if ((t & (SEG - 1)) == 0) {
+            if (self) {
+                out[0] = a;
+                out[m2 / 2] = b;
+            } else {
+                out[r1] = a + b;
+            }
+        }
     } else {
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (l == 0) red[t >> 6] = acc;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o);
+            b += __shfl_xor(b, o);
+        }
+        if (l == 0) {
+            red[t >> 6] = a;
+            red[FFT_THREADS / 64 + (t >> 6)] = b;
+        }
         __syncthreads();
-        if (t < P && p0 + (uint32_t)t < npairs) {
-            double sum = 0.0;
+        if (t < P) {
+            double sa = 0.0, sb = 0.0;
 #pragma unroll
-            for (int k = 0; k < SEG / 64; k++) sum += red[t * (SEG / 64) + k];
-            out[t] = sum;
+            for (int k = 0; k < SEG / 64; k++) {
+                sa += red[t * (SEG / 64) + k];
+                sb += red[FFT_THREADS / 64 + t * (SEG / 64) + k];
+            }
+            if (p0 == 0 && t == 0) {
+                out[0] = sa;
+                out[m2 / 2] = sb;
+            } else {
+                out[p0 + t] = sa + sb;
+            }
         }
     }
 }
@@ -1058,11 +1089,11 @@ static void launch_pass2(hipStream_t s, const SpecGeom& g, const uint32_t* trial
                          const float2* work, void* spec, double* partial, const float2* twrow,
                          const float* cos_tables) {
     if constexpr (M1 == 4096) {
-        hipLaunchKernelGGL((k_fft_pass2r<F16>), dim3(g.m2 / 2 + 1, ntrials), dim3(FFT_THREADS), 0,
+        hipLaunchKernelGGL((k_fft_pass2r<F16>), dim3(g.m2 / 2, ntrials), dim3(FFT_THREADS), 0,
                            s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial, twrow, cos_tables);
     } else if constexpr (M1 >= 256) {
-        constexpr uint32_t P = 8192 / M1 / 2;
-        hipLaunchKernelGGL((k_fft_pass2x<M1 / 256, F16>), dim3((g.m2 / 2 + 1 + P - 1) / P, ntrials),
+        constexpr uint32_t P = 8192 / M1 / 2;  // divides m2 / 2 (>= 256)
+        hipLaunchKernelGGL((k_fft_pass2x<M1 / 256, F16>), dim3(g.m2 / 2 / P, ntrials),
                            dim3(FFT_THREADS), 0, s, work, g.n, trial_n, g.m, g.m2, spec, g.spec_pitch, partial,
                            twrow, cos_tables);
     } else {
