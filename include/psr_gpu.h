@@ -221,6 +221,13 @@ int psr_periodicity_noalloc(psr_ctx *ctx, const float *series, size_t n, double 
  * them; raw then holds the power a * a the whitening and scoring use. */
 int psr_spectrum(psr_ctx *ctx, const float *series, size_t n, double tsamp,
                  const psr_search_cfg *cfg, float *raw, float *white, size_t *nbins);
+/* No counterpart in the reference.  psr_spectrum with the four-step FFT split
+ * forced to m2 rows of m1 = next_pow2(n) / m2 (m2 = 512, 1024 or 2048; 0 = the
+ * default split): the splits psr_batch's autotuner chooses among, for checks
+ * of each against the oracle.  PSR_E_SHAPE for an m2 with no split at this
+ * length (spectra of at most 4096 points are one pass). */
+int psr_spectrum_split(psr_ctx *ctx, const float *series, size_t n, double tsamp,
+                       const psr_search_cfg *cfg, uint32_t m2, float *raw, float *white, size_t *nbins);
 
 /* ---- many observations ------------------------------------------------- */
 /* The reference searches one observation per `psr search` run (the per-trial

@@ -156,6 +156,7 @@ EXPORTS = [
                                           C.POINTER(SearchCfg), C.POINTER(Candidate), C.c_size_t,
                                           C.POINTER(C.c_size_t)]),
     ("psr_spectrum", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(SearchCfg), C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
+    ("psr_spectrum_split", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(SearchCfg), C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_size_t)]),
     ("psr_fold", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     ("psr_refine", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_size_t, C.c_double, C.c_double, C.c_double, C.c_size_t, C.c_uint32, C.POINTER(ProfileStats), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
     ("psr_batch_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(C.POINTER(C.c_double)),

@@ -985,8 +985,8 @@ int psr_periodicity(psr_ctx* c, const float* series, size_t n, double tsamp, dou
     return PSR_OK;
 }
 
-int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp,
-                 const psr_search_cfg* cfg, float* raw, float* white, size_t* nbins) {
+static int spectrum_split(psr_ctx* c, const float* series, size_t n, double tsamp, const psr_search_cfg* cfg,
+                          uint32_t m2, float* raw, float* white, size_t* nbins) {
     clear_error();
     if (!c || !series || !cfg || !nbins) return fail(PSR_E_INVALID, "null pointer");
     if (n < 2 || !std::isfinite(tsamp) || tsamp <= 0.0)
@@ -994,7 +994,13 @@ int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp,
     if (n > (1u << 23)) return fail(PSR_E_SHAPE, "series longer than 2^23 samples is not supported");
     int rc;
     if ((rc = check_device(c))) return rc;
-    const SpecGeom g = make_geometry((uint32_t)n);
+    SpecGeom g = make_geometry((uint32_t)n);
+    if (m2) {  // the four-step split with m2 rows (what the batch autotuner may pick)
+        if (m2 != 512 && m2 != 1024 && m2 != 2048) return fail(PSR_E_SHAPE, "m2 must be 0, 512, 1024 or 2048");
+        g = make_geometry_split((uint32_t)n, m2);
+        if (g.m1 == 1 || g.m2 != m2 || !fft_shape_supported(g))
+            return fail(PSR_E_SHAPE, "no four-step split with these m2 rows for this length");
+    }
     *nbins = g.nbins;
     if (!raw && !white) return PSR_OK;
     psr_search_cfg cf = *cfg;
@@ -1037,6 +1043,16 @@ int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp,
             for (uint32_t k = birdies[b]; k <= birdies[b + 1]; k++) white[k] = 0.0f;
     }
     return PSR_OK;
+}
+
+int psr_spectrum(psr_ctx* c, const float* series, size_t n, double tsamp, const psr_search_cfg* cfg,
+                 float* raw, float* white, size_t* nbins) {
+    return spectrum_split(c, series, n, tsamp, cfg, 0, raw, white, nbins);
+}
+
+int psr_spectrum_split(psr_ctx* c, const float* series, size_t n, double tsamp, const psr_search_cfg* cfg,
+                       uint32_t m2, float* raw, float* white, size_t* nbins) {
+    return spectrum_split(c, series, n, tsamp, cfg, m2, raw, white, nbins);
 }
 
 // ------------------------------------------------------------------- fold

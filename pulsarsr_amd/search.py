@@ -74,20 +74,27 @@ def periodicity_noalloc(series, tsamp, dm, config=None, ctx=None):
     return _cands(out, n.value)
 
 
-def spectrum(series, tsamp, config=None, ctx=None):
-    """(raw |X|^2, whitened power) of periodicity's spectral stages (lib.rs:35-72)."""
+def spectrum(series, tsamp, config=None, ctx=None, m2=0):
+    """(raw |X|^2, whitened power) of periodicity's spectral stages (lib.rs:35-72).
+    m2 = 512, 1024 or 2048 forces that four-step FFT split (psr_spectrum_split)."""
     config = config or Config()
     ctx = ctx or default_context()
     x = np.ascontiguousarray(series, dtype=np.float32)
     nb = C.c_size_t()
     cfg, keep = config._c()
-    check(_ffi.lib().psr_spectrum(ctx.handle, x.ctypes.data_as(C.POINTER(C.c_float)), len(x), float(tsamp),
-                                  C.byref(cfg), None, None, C.byref(nb)))
+    lib = _ffi.lib()
+
+    def call(raw, white):
+        xp = x.ctypes.data_as(C.POINTER(C.c_float))
+        if m2:
+            return lib.psr_spectrum_split(ctx.handle, xp, len(x), float(tsamp), C.byref(cfg), int(m2), raw, white,
+                                          C.byref(nb))
+        return lib.psr_spectrum(ctx.handle, xp, len(x), float(tsamp), C.byref(cfg), raw, white, C.byref(nb))
+
+    check(call(None, None))
     raw = np.zeros(nb.value, dtype=np.float32)
     white = np.zeros(nb.value, dtype=np.float32)
-    check(_ffi.lib().psr_spectrum(ctx.handle, x.ctypes.data_as(C.POINTER(C.c_float)), len(x), float(tsamp),
-                                  C.byref(cfg), raw.ctypes.data_as(C.POINTER(C.c_float)),
-                                  white.ctypes.data_as(C.POINTER(C.c_float)), C.byref(nb)))
+    check(call(raw.ctypes.data_as(C.POINTER(C.c_float)), white.ctypes.data_as(C.POINTER(C.c_float))))
     del keep
     return raw, white
 

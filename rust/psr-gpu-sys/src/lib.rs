@@ -243,6 +243,9 @@ extern "C" {
                                    count: *mut usize) -> c_int;
     pub fn psr_spectrum(ctx: *mut psr_ctx, series: *const f32, n: usize, tsamp: f64, cfg: *const psr_search_cfg,
                         raw: *mut f32, white: *mut f32, nbins: *mut usize) -> c_int;
+    pub fn psr_spectrum_split(ctx: *mut psr_ctx, series: *const f32, n: usize, tsamp: f64,
+                              cfg: *const psr_search_cfg, m2: u32, raw: *mut f32, white: *mut f32,
+                              nbins: *mut usize) -> c_int;
 
     pub fn psr_batch_create(ctx: *mut psr_ctx, obs: *const *mut psr_obs, nobs: usize, dms: *const *const f64,
                             ntrials: *const usize, lengths: *const usize, cfg: *const psr_search_cfg,

@@ -244,6 +244,37 @@ def test_spectrum_matches_oracle(ctx, n):
     assert np.linalg.norm(white - owhite) <= 1e-5 * np.linalg.norm(owhite)
 
 
+# The four-step splits psr_batch's autotuner chooses among (m2 = 1024 and 2048
+# rows of m1 = M / m2): row lengths m1 = 32 .. 2048 with the other pass-1
+# lengths, FP32 against the oracle as above; FP16 storage (amplitudes |X| /
+# sqrt(n): n times the power it returns) normwise within its half-precision
+# rounding (5e-3 of the power, the bound the configs hold it to)
+@pytest.mark.parametrize("n,m2", [(50000, 1024), (50000, 2048), (200000, 1024), (200000, 2048), (263500, 2048),
+                                  (600000, 1024), (600000, 2048), (1500000, 1024), (1500000, 2048),
+                                  (2300000, 2048)])
+def test_spectrum_splits_match_oracle(ctx, n, m2):
+    print("split n=%d m2=%d" % (n, m2), flush=True)
+    rng = np.random.default_rng(n + m2)
+    t = np.arange(n)
+    x = (rng.standard_normal(n) * 16 + 2000 + 30 * np.sin(2 * np.pi * t / 97.3)).round().astype(np.float32)
+    oraw, owhite = oracle.spectrum(x, 64e-6, min_hz=10.0)
+    raw, white = search.spectrum(x, 64e-6, search.Config(min_hz=10.0), m2=m2)
+    assert len(raw) == len(oraw)
+    assert np.linalg.norm(raw - oraw) <= 1e-5 * np.linalg.norm(oraw)
+    assert np.max(np.abs(white - owhite)) <= 1e-4 * max(1.0, np.max(np.abs(owhite))) + 1e-5
+    raw16, _ = search.spectrum(x, 64e-6, search.Config(min_hz=10.0, precision=PRECISION_FP16), m2=m2)
+    p16 = raw16.astype(np.float64) * n
+    assert np.linalg.norm(p16 - oraw) <= 5e-3 * np.linalg.norm(oraw)
+
+
+def test_spectrum_split_errors(ctx):
+    x = np.zeros(600000, dtype=np.float32)
+    with pytest.raises(PsrError, match="m2 must be"):
+        search.spectrum(x, 64e-6, search.Config(min_hz=10.0), m2=777)
+    with pytest.raises(PsrError, match="no four-step split"):  # 4096 points: one pass
+        search.spectrum(x[:4000], 64e-6, search.Config(min_hz=10.0), m2=512)
+
+
 @pytest.mark.parametrize("n", [263500, 1500000, 2097152, 4194305, 8388608])
 def test_fp16_whitening_scales_are_exact_window_medians(ctx, n):
     # FP16 storage (packed 16-bit block sort, windowed cut search): every 64-bin
