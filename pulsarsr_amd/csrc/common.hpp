@@ -168,10 +168,11 @@ constexpr int TW4096_OFFSET = (COSTAB_SIZE + 1) & ~1;
 constexpr int COSTAB_ALL = TW4096_OFFSET + 2 * 15 * 256;
 void fill_costabs(float* host_table);  // COSTAB_ALL floats
 // Intermediate bytes per four-step FFT batch (the work buffer holds this many
-// trials' intermediates): 192 MiB (11 trials at 2^22) stays within the 256 MiB
-// Infinity Cache between the two passes; measured against 128 MiB to 1 GiB
-// batches it gives the shortest headline step.
-constexpr uint64_t FFT_BATCH_BYTES = 192ull << 20;
+// trials' intermediates): 224 MiB (14 trials of 2^21 points, 3584 workgroups
+// per pass: whole rounds of two per CU) stays within the 256 MiB Infinity
+// Cache between the two passes; measured against 128 MiB to 1 GiB batches it
+// gives the shortest headline step.
+constexpr uint64_t FFT_BATCH_BYTES = 224ull << 20;
 uint32_t fft_work_trials(const SpecGeom& g, uint32_t ntrials);
 // true when launch_fft has kernels for the geometry's split (m2 of 512, 1024 or
 // 2048 in the four-step form; every power of two up to 4096 in one pass)
