@@ -249,6 +249,14 @@ template <typename KeyT, int STRIDE>
 constexpr uint32_t key_blocks_elems(uint32_t nb) {  // elements of nb staged blocks
     return nb * (uint32_t)STRIDE + ((nb + 7) >> 3) * (uint32_t)(4 / sizeof(KeyT));
 }
+// 16-bit keys: the two words after each block's 64 keys hold 0xFFFF, the value
+// every key past a block's count has, which the window median's reads rely on
+// (threads t of nt fill them; the block stores never touch them)
+template <int STRIDE>
+__device__ __forceinline__ void pad_key_blocks(uint16_t* s, uint32_t nb, uint32_t t, uint32_t nt) {
+    static_assert(STRIDE >= 66, "two pad words per block");
+    for (uint32_t i = t; i < 2 * nb; i += nt) s[key_block<uint16_t, STRIDE>(i >> 1) + 64 + (i & 1)] = 0xFFFFu;
+}
 
 // Whitening scale of block b (lib.rs:45-60) from the sorted blocks b-2 .. b+2
 // at s + key_block(lb + j): the exact window median is found by a cut search --
@@ -313,56 +321,105 @@ __device__ float window_scale(const KeyT* s, uint32_t lb, uint32_t b, uint32_t n
     // arithmetically, the eight keys next to their cuts are read together
     // (indices clamped into the block, out-of-range keys replaced by the
     // sentinels), and the five cut states are updated with selects.
-    for (int it = 0; it < 1024; it++) {
-        int jl = 0, ju = 0;
-        uint32_t lmax = L[0], umin = U[0];
-#pragma unroll
-        for (int j = 1; j < 5; j++) {
-            if (L[j] > lmax) {
-                lmax = L[j];
-                jl = j;
-            }
-            if (U[j] < umin) {
-                umin = U[j];
-                ju = j;
-            }
-        }
-        if (lmax <= umin) break;
-        int pl = p[0], pu = p[0], nu = nn[0];
-#pragma unroll
-        for (int j = 1; j < 5; j++) {
-            pl = j == jl ? p[j] : pl;
-            pu = j == ju ? p[j] : pu;
-            nu = j == ju ? nn[j] : nu;
-        }
-        // jl != ju (within one sorted block L <= U)
-        const KeyT* Al = s + key_block<KeyT, STRIDE>(lb + jl);
-        const KeyT* Au = s + key_block<KeyT, STRIDE>(lb + ju);
-        auto below = [](int i) { return i < 0 ? 0 : i; };
-        auto above = [](int i) { return i > 63 ? 63 : i; };
-        const uint32_t a2 = Al[below(pl - 2)], a3 = Al[below(pl - 3)], a4 = Al[below(pl - 4)],
-                       a5 = Al[below(pl - 5)];
-        const uint32_t b1 = Au[above(pu + 1)], b2 = Au[above(pu + 2)], b3 = Au[above(pu + 3)],
-                       b4 = Au[above(pu + 4)];
-        const bool s4 = pl >= 4 && pu + 4 <= nu && a4 > b3;
-        const bool s2 = pl >= 2 && pu + 2 <= nu && a2 > b1;
-        const int st = s4 ? 4 : (s2 ? 2 : 1);
-        // jl keeps keys [0, pl - st): new L = key pl - st - 1, new U = key pl - st
-        const uint32_t kl = s4 ? a5 : (s2 ? a3 : a2);
-        const uint32_t newLl = pl - st >= 1 ? kl + 1u : 0u;
-        const uint32_t newUl = s4 ? a4 + 1u : (s2 ? a2 + 1u : lmax);
-        // ju's left part grows to [0, pu + st): new L = key pu + st - 1, new U = key pu + st
-        const uint32_t newLu = s4 ? b3 + 1u : (s2 ? b1 + 1u : umin);
-        const uint32_t ku = s4 ? b4 : (s2 ? b2 : b1);
-        const uint32_t newUu = pu + st < nu ? ku + 1u : KEY_INF;
+    if constexpr (F16) {
+        // 16-bit keys: each block's state packed as (bound << 10 | p << 3 | j), so a
+        // max3/min3 pair picks the block (ties by p, then j: any choice ends at the
+        // same median) and carries its cut; U's infinity is 0x10000, which is also
+        // key + 1 for every stored key past a block's count (0xFFFF, pads included),
+        // so the reads need no count checks: a key past the count never exceeds
+        // the other side, and a new U there is infinite
+        uint32_t Lp[5], Up[5];
 #pragma unroll
         for (int j = 0; j < 5; j++) {
-            const bool isl = j == jl, isu = j == ju;
-            p[j] += isu ? st : (isl ? -st : 0);
-            const uint32_t l2 = isl ? newLl : (isu ? newLu : L[j]);
-            const uint32_t u2 = isl ? newUl : (isu ? newUu : U[j]);
-            L[j] = l2;
-            U[j] = u2;
+            Lp[j] = L[j] << 10 | (uint32_t)p[j] << 3 | (uint32_t)j;
+            Up[j] = (U[j] == KEY_INF ? 0x10000u : U[j]) << 10 | (uint32_t)p[j] << 3 | (uint32_t)j;
+        }
+        for (int it = 0; it < 1024; it++) {
+            const uint32_t lm = max(max(max(Lp[0], Lp[1]), Lp[2]), max(Lp[3], Lp[4]));
+            const uint32_t um = min(min(min(Up[0], Up[1]), Up[2]), min(Up[3], Up[4]));
+            if ((lm >> 10) <= (um >> 10)) break;
+            const uint32_t jl = lm & 7u, ju = um & 7u;
+            const int pl = (int)((lm >> 3) & 127u), pu = (int)((um >> 3) & 127u);
+            const KeyT* Al = s + key_block<KeyT, STRIDE>(lb + jl);
+            const KeyT* Au = s + key_block<KeyT, STRIDE>(lb + ju);
+            auto below = [](int i) { return i < 0 ? 0 : i; };
+            auto above = [](int i) { return i > 65 ? 65 : i; };
+            const uint32_t a2 = Al[below(pl - 2)], a3 = Al[below(pl - 3)], a4 = Al[below(pl - 4)],
+                           a5 = Al[below(pl - 5)];
+            const uint32_t b1 = Au[above(pu + 1)], b2 = Au[above(pu + 2)], b3 = Au[above(pu + 3)],
+                           b4 = Au[above(pu + 4)];
+            const bool s4 = pl >= 4 && a4 > b3;
+            const bool s2 = pl >= 2 && a2 > b1;
+            const int st = s4 ? 4 : (s2 ? 2 : 1);
+            const uint32_t kl = s4 ? a5 : (s2 ? a3 : a2);
+            const uint32_t newLl = pl - st >= 1 ? kl + 1u : 0u;
+            const uint32_t newUl = s4 ? a4 + 1u : (s2 ? a2 + 1u : lm >> 10);
+            const uint32_t newLu = s4 ? b3 + 1u : (s2 ? b1 + 1u : um >> 10);
+            const uint32_t newUu = (s4 ? b4 : (s2 ? b2 : b1)) + 1u;
+            const uint32_t ql = (uint32_t)(pl - st) << 3 | jl, qu = (uint32_t)(pu + st) << 3 | ju;
+            const uint32_t lpl = newLl << 10 | ql, upl = newUl << 10 | ql;
+            const uint32_t lpu = newLu << 10 | qu, upu = newUu << 10 | qu;
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                const bool isl = (uint32_t)j == jl, isu = (uint32_t)j == ju;
+                Lp[j] = isl ? lpl : (isu ? lpu : Lp[j]);
+                Up[j] = isl ? upl : (isu ? upu : Up[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 5; j++) U[j] = Up[j] >> 10;
+    } else {
+        for (int it = 0; it < 1024; it++) {
+            int jl = 0, ju = 0;
+            uint32_t lmax = L[0], umin = U[0];
+#pragma unroll
+            for (int j = 1; j < 5; j++) {
+                if (L[j] > lmax) {
+                    lmax = L[j];
+                    jl = j;
+                }
+                if (U[j] < umin) {
+                    umin = U[j];
+                    ju = j;
+                }
+            }
+            if (lmax <= umin) break;
+            int pl = p[0], pu = p[0], nu = nn[0];
+#pragma unroll
+            for (int j = 1; j < 5; j++) {
+                pl = j == jl ? p[j] : pl;
+                pu = j == ju ? p[j] : pu;
+                nu = j == ju ? nn[j] : nu;
+            }
+            // jl != ju (within one sorted block L <= U)
+            const KeyT* Al = s + key_block<KeyT, STRIDE>(lb + jl);
+            const KeyT* Au = s + key_block<KeyT, STRIDE>(lb + ju);
+            auto below = [](int i) { return i < 0 ? 0 : i; };
+            auto above = [](int i) { return i > 63 ? 63 : i; };
+            const uint32_t a2 = Al[below(pl - 2)], a3 = Al[below(pl - 3)], a4 = Al[below(pl - 4)],
+                           a5 = Al[below(pl - 5)];
+            const uint32_t b1 = Au[above(pu + 1)], b2 = Au[above(pu + 2)], b3 = Au[above(pu + 3)],
+                           b4 = Au[above(pu + 4)];
+            const bool s4 = pl >= 4 && pu + 4 <= nu && a4 > b3;
+            const bool s2 = pl >= 2 && pu + 2 <= nu && a2 > b1;
+            const int st = s4 ? 4 : (s2 ? 2 : 1);
+            // jl keeps keys [0, pl - st): new L = key pl - st - 1, new U = key pl - st
+            const uint32_t kl = s4 ? a5 : (s2 ? a3 : a2);
+            const uint32_t newLl = pl - st >= 1 ? kl + 1u : 0u;
+            const uint32_t newUl = s4 ? a4 + 1u : (s2 ? a2 + 1u : lmax);
+            // ju's left part grows to [0, pu + st): new L = key pu + st - 1, new U = key pu + st
+            const uint32_t newLu = s4 ? b3 + 1u : (s2 ? b1 + 1u : umin);
+            const uint32_t ku = s4 ? b4 : (s2 ? b2 : b1);
+            const uint32_t newUu = pu + st < nu ? ku + 1u : KEY_INF;
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                const bool isl = j == jl, isu = j == ju;
+                p[j] += isu ? st : (isl ? -st : 0);
+                const uint32_t l2 = isl ? newLl : (isu ? newLu : L[j]);
+                const uint32_t u2 = isl ? newUl : (isu ? newUu : U[j]);
+                L[j] = l2;
+                U[j] = u2;
+            }
         }
     }
     uint32_t umin = U[0];
@@ -387,6 +444,7 @@ __global__ __launch_bounds__(MB_BLOCKS) void k_block_median(
     using KeyT = typename std::conditional<F16, uint16_t, uint32_t>::type;
     constexpr int STRIDE = F16 ? 2 * 33 : (int)MB_STRIDE;
     __shared__ KeyT s[key_blocks_elems<KeyT, STRIDE>(MB_BLOCKS + 4)];
+    if constexpr (F16) pad_key_blocks<STRIDE>(s, MB_BLOCKS + 4, threadIdx.x, MB_BLOCKS);
     constexpr int E = F16 ? 8 : 4;                                  // keys per 16-byte load
     constexpr int NV = (int)((MB_BLOCKS + 4) * 64 / E);             // vectors staged
     constexpr int VPT = (NV + (int)MB_BLOCKS - 1) / (int)MB_BLOCKS;  // per thread
@@ -463,6 +521,7 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
     KeyT* sk = (KeyT*)lds_w;             // sorted blocks, local block lb = global - b0 + 2
     uint32_t* side = lds_w + MAIN / 4;  // [4][64]: blocks b0-2, b0-1, b0+SB, b0+SB+1
     __shared__ float sfloor;
+    if constexpr (F16) pad_key_blocks<STRIDE>(sk, SB_BLOCKS + 6, threadIdx.x, 256);
     const uint2 wb = xcd_block();
     const uint32_t trial = wb.y;
     const uint32_t k1a = wb.x * cw;
@@ -567,6 +626,7 @@ __global__ __launch_bounds__(256) void k_whiten_fused16(
     uint16_t* sk = (uint16_t*)lds_w;     // sorted blocks, local block lb = global - b0 + 2
     uint32_t* side = lds_w + MAIN / 4;  // [4][64]: blocks b0-2, b0-1, b0+SB, b0+SB+1
     __shared__ float sfloor;
+    pad_key_blocks<STRIDE>(sk, SB_BLOCKS + 6, threadIdx.x, 256);
     const uint2 wb = xcd_block();
     const uint32_t trial = wb.y;
     const uint32_t ngroups = m1 / cw;
