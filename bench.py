@@ -84,10 +84,10 @@ def top_families(cands, n, tsamp, k=20):
 # that launch shape only: other workloads or trial counts report traffic null.
 MEASURED_LAUNCH = {"workload": "headline-1024ch-8bit", "trials": 4000, "launches": 1}
 MEASURED_TRAFFIC_GB_PER_LAUNCH = {
-    "k_dedisp_u8": 85.71 + 17.64,
-    "k_fft_pass1r": 17.03 + 67.11,
+    "k_dedisp_u8": 85.65 + 17.64,
+    "k_fft_pass1r": 17.02 + 67.11,
     "k_fft_pass2r": 67.25 + 16.81,
-    "k_whiten_fused16": 31.90 + 17.36,
+    "k_whiten_fused16": 32.54 + 17.36,
     "k_harmsum": 15.22 + 0.02,
 }
 
