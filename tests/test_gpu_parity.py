@@ -275,19 +275,21 @@ def test_spectrum_split_errors(ctx):
         search.spectrum(x[:4000], 64e-6, search.Config(min_hz=10.0), m2=512)
 
 
-@pytest.mark.parametrize("n", [263500, 1500000, 2097152, 4194305, 8388608])
-def test_fp16_whitening_scales_are_exact_window_medians(ctx, n):
+# (columns per whitening group: 32 at m2 = 512, 16 at m2 = 1024, 8 at m2 = 2048)
+@pytest.mark.parametrize("n,m2", [(263500, 0), (1500000, 0), (2097152, 0), (2300000, 0), (4194305, 0),
+                                  (8388608, 0), (600000, 1024)])
+def test_fp16_whitening_scales_are_exact_window_medians(ctx, n, m2):
     # FP16 storage (packed 16-bit block sort, windowed cut search): every 64-bin
     # block's scale is max(max(upper median / ln2, floor), 1e-20) of the stored
     # powers a * a over bins [64b - 128, 64b + 192) clipped to [1, nbins)
     # (crates/search/src/lib.rs:45-60), so the whitened power must equal
     # raw / scale recomputed from the returned raw spectrum with the oracle's
     # median, bit for bit (the floor, mean * 1e-6, is far below every median here)
-    print("fp16 whitening n=%d" % n, flush=True)
+    print("fp16 whitening n=%d m2=%d" % (n, m2), flush=True)
     rng = np.random.default_rng(n + 7)
     t = np.arange(n)
     x = (rng.standard_normal(n) * 16 + 2000 + 30 * np.sin(2 * np.pi * t / 97.3)).round().astype(np.float32)
-    raw, white = search.spectrum(x, 64e-6, search.Config(min_hz=10.0, precision=PRECISION_FP16))
+    raw, white = search.spectrum(x, 64e-6, search.Config(min_hz=10.0, precision=PRECISION_FP16), m2=m2)
     nb = len(raw)
     assert np.all(np.isfinite(raw)) and raw[1:].min() >= 0.0
     floor = np.float32(raw.astype(np.float64).sum() / nb * 1e-6)
