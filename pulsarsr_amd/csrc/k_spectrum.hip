@@ -610,12 +610,12 @@ __global__ __launch_bounds__(256) void k_whiten_fused(
             window_scale<F16, KeyT, STRIDE>(sk, SB_BLOCKS, nsorted, nblocks, nbins, fl);
 }
 
-// FP16 form of k_whiten_fused for two column groups per workgroup: the keys
-// stay packed two to a register, (key i, key i + 32), from the loads on, and
-// the second group's 64 loads per lane are issued as soon as the first group's
-// keys are in the sorting registers, so their latency overlaps the first
-// group's sort, LDS exchange and window medians.
-constexpr uint32_t WH_GROUPS = 2;
+// FP16 form of k_whiten_fused for WH_GROUPS column groups per workgroup: the
+// keys stay packed two to a register, (key i, key i + 32), from the loads on,
+// and the next group's 64 loads per lane are issued as soon as the current
+// group's keys are in the sorting registers, so their latency overlaps its
+// sort, LDS exchange and window medians.
+constexpr uint32_t WH_GROUPS = 4;
 __global__ __launch_bounds__(256) void k_whiten_fused16(
     const uint16_t* __restrict__ spec_t, uint32_t m, uint32_t m1, uint32_t m2, uint32_t cw,
     uint32_t nbins, uint32_t nblocks, uint64_t spec_pitch, const double* __restrict__ partial,
